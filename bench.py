@@ -1,0 +1,170 @@
+"""Benchmark: FP64 matrix-free A11 MatMult (MatMult_MFStokes_A11) on one MI355X.
+
+Workload: 64^3 Q2 elements (6,440,067 velocity dofs), affine unit cube, sinker
+viscosity (32 seeded spheres of radius 0.05-0.12 at 1e4 in a background of 1,
+evaluated at the 27 Gauss points per element), random input velocity with
+Dirichlet rows on all faces.  One step = one MatMult with vectors resident in
+HBM (column masking, element apply, Dirichlet rows).  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--m 64] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOPS_PER_ELEMENT = 17253  # stokes_operators_tensor.c:281-283
+PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--m", type=int, default=64, help="elements per direction")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}), flush=True)
+        return 2
+
+    from ptatin3d_amd import fields as F
+    from ptatin3d_amd._lib import DeviceArray, device_count
+    from ptatin3d_amd.operators import MatA11MF
+
+    if device_count() < 1:
+        print(json.dumps({"error": "no GPU visible"}), flush=True)
+        return 1
+
+    m = args.m
+    c = F.config2(m)
+    E = m ** 3
+    N = F.nnodes(m, m, m)
+    ndof = 3 * N
+    A = MatA11MF(m, m, m).setup(c["coords"], c["eta"], c["bc"])
+    x = DeviceArray.from_host(c["u"])
+    y = DeviceArray(ndof)
+
+    # warmup, then the timed region: K MatMults bracketed by device syncs
+    if args.warmup > 0:
+        A.time_mult(x, y, args.warmup)
+    from ptatin3d_amd._lib import call
+    call("ptmf_device_synchronize")
+    t0 = time.perf_counter()
+    ms_event = A.time_mult(x, y, args.steps)
+    call("ptmf_device_synchronize")
+    t1 = time.perf_counter()
+    ms_wall = 1e3 * (t1 - t0) / args.steps
+    ms_step = max(ms_wall, ms_event)
+    y_gpu = y.download()
+
+    # dominant kernel alone (HIP events on the operator's stream)
+    ms_kernel = A.time_kernel(x, y, args.steps)
+    flops = FLOPS_PER_ELEMENT * E
+    alg_bytes = 8 * (9 * N + 27 * E)
+    tflops = flops / (ms_kernel * 1e-3) / 1e12
+    gbs = alg_bytes / (ms_kernel * 1e-3) / 1e9
+
+    # diagonal (setup-time companion of the apply in this workload)
+    d = DeviceArray(ndof)
+    t2 = time.perf_counter()
+    A.get_diagonal(d)
+    ms_diag = 1e3 * (time.perf_counter() - t2)
+
+    gdofs = ndof / (ms_step * 1e-3) / 1e9
+    out = {
+        "metric": "MF A11 GDoF/s (% roofline); time to 1e-8 Stokes solve, FP32-GMG vs FP64",
+        "value": round(gdofs, 4),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded mesh, sinker viscosity, random velocity)",
+        "config": {
+            "workload": f"FP64 A11 MatMult, {m}^3 Q2 elements, affine unit cube, sinker viscosity "
+                        "(32 spheres at 1e4), all-face Dirichlet",
+            "elements": E,
+            "velocity_dofs": ndof,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "peak_kind": "FP64 78.6 TFLOP/s (vector FMA = matrix peak on MI355X; the kernel uses VALU FMA)",
+            "achieved": round(tflops, 3),
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(tflops / PEAK_FP64_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "a11_apply_kernel<double>",
+            "kernel_ms": round(ms_kernel, 5),
+            "alg_flops_per_launch": flops,
+            "alg_bytes_per_launch": alg_bytes,
+            "alg_hbm_GBs": round(gbs, 1),
+            "hbm_frac": round(gbs / PEAK_HBM_GBS, 4),
+        },
+        "timing": {"ms_event": round(ms_event, 5), "ms_wall": round(ms_wall, 5),
+                   "ms_diagonal": round(ms_diag, 3)},
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        threads = O.num_threads()
+        t0 = time.perf_counter()
+        nrun = 0
+        y_cpu = None
+        while True:
+            y_cpu = O.a11_matmult(m, m, m, c["coords"], c["eta"], c["bc"], c["u"])
+            nrun += 1
+            el = time.perf_counter() - t0
+            if el >= args.cpu_seconds or nrun >= 50:
+                break
+        cpu_s = el / nrun
+        out["cpu_baseline"] = {
+            "value": round(ndof / cpu_s / 1e9, 5),
+            "unit": "GDoF/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{nrun} FP64 A11 MatMults of the same {m}^3 workload "
+                      f"({el:.1f} s) on {threads} OpenMP threads, {cpu_model()}",
+        }
+        err = float(np.abs(y_gpu - y_cpu).max() / np.abs(y_cpu).max())
+        out["parity_rel_inf_vs_oracle"] = err
+
+    print(json.dumps(out), flush=True)
+    A.destroy()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,135 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker -- never by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIBPATH
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        _lib = C.CDLL(LIBPATH)
+        _lib.orc_num_threads.restype = C.c_int
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def num_threads() -> int:
+    return load().orc_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    load().orc_set_num_threads(C.c_int(n))
+
+
+def nn(mx, my, mz):
+    return (2 * mx + 1) * (2 * my + 1) * (2 * mz + 1)
+
+
+def a11_tensor(mx, my, mz, coords, eta, u, y=None):
+    """y += A11 u (no BCs), tensor path."""
+    y = np.zeros(3 * nn(mx, my, mz)) if y is None else y
+    load().orc_a11_tensor(mx, my, mz, _p(_f64(coords)), _p(_f64(eta)), _p(_f64(u)), _p(y))
+    return y
+
+
+def a11_ref(mx, my, mz, coords, eta, u, y=None):
+    y = np.zeros(3 * nn(mx, my, mz)) if y is None else y
+    load().orc_a11_ref(mx, my, mz, _p(_f64(coords)), _p(_f64(eta)), _p(_f64(u)), _p(y))
+    return y
+
+
+def a11_element_matrix(mx, my, mz, coords, eta, e):
+    Ae = np.zeros((81, 81))
+    load().orc_a11_element_matrix(mx, my, mz, _p(_f64(coords)), _p(_f64(eta)), C.c_long(e), _p(Ae))
+    return Ae
+
+
+def a11_asm_matmult(mx, my, mz, coords, eta, bc, x):
+    y = np.zeros(3 * nn(mx, my, mz))
+    load().orc_a11_asm_matmult(mx, my, mz, _p(_f64(coords)), _p(_f64(eta)), _p(_u8(bc)), _p(_f64(x)), _p(y))
+    return y
+
+
+def a11_matmult(mx, my, mz, coords, eta, bc, x):
+    y = np.zeros(3 * nn(mx, my, mz))
+    load().orc_a11_matmult(mx, my, mz, _p(_f64(coords)), _p(_f64(eta)), _p(_u8(bc)), _p(_f64(x)), _p(y))
+    return y
+
+
+def a11_diagonal(mx, my, mz, coords, eta, bc):
+    d = np.zeros(3 * nn(mx, my, mz))
+    load().orc_a11_diagonal(mx, my, mz, _p(_f64(coords)), _p(_f64(eta)), _p(_u8(bc)), _p(d))
+    return d
+
+
+def a12_matmult(mx, my, mz, coords, bc, p):
+    y = np.zeros(3 * nn(mx, my, mz))
+    load().orc_a12_matmult(mx, my, mz, _p(_f64(coords)), _p(_u8(bc)), _p(_f64(p)), _p(y))
+    return y
+
+
+def a21_matmult(mx, my, mz, coords, bc, u):
+    y = np.zeros(4 * mx * my * mz)
+    load().orc_a21_matmult(mx, my, mz, _p(_f64(coords)), _p(_u8(bc)), _p(_f64(u)), _p(y))
+    return y
+
+
+def schur_mass_blocks(mx, my, mz, coords, eta):
+    M = np.zeros(16 * mx * my * mz)
+    load().orc_schur_mass_blocks(mx, my, mz, _p(_f64(coords)), _p(_f64(eta)), _p(M))
+    return M.reshape(-1, 4, 4)
+
+
+def prolong_add(mxc, myc, mzc, xc, yf):
+    load().orc_prolong_add(mxc, myc, mzc, _p(_f64(xc)), _p(yf))
+    return yf
+
+
+def restrict_add(mxc, myc, mzc, xf, yc):
+    load().orc_restrict_add(mxc, myc, mzc, _p(_f64(xf)), _p(yc))
+    return yc
+
+
+def inject_coords(mxc, myc, mzc, xf):
+    xc = np.zeros(3 * nn(mxc, myc, mzc))
+    load().orc_inject_coords(mxc, myc, mzc, _p(_f64(xf)), _p(xc))
+    return xc
+
+
+def cheb_jacobi(mx, my, mz, coords, eta, bc, diag, emin, emax, its, b, x0=None):
+    x = np.zeros(3 * nn(mx, my, mz)) if x0 is None else np.array(x0, dtype=np.float64)
+    load().orc_cheb_jacobi(mx, my, mz, _p(_f64(coords)), _p(_f64(eta)), _p(_u8(bc)), _p(_f64(diag)),
+                           C.c_double(emin), C.c_double(emax), C.c_int(its),
+                           C.c_int(0 if x0 is None else 1), _p(_f64(b)), _p(x))
+    return x

@@ -1,0 +1,153 @@
+"""ctypes binding of the ptmf C-ABI (include/ptmf.h).
+
+The product path is libptmf.so only: if it is missing or has no GPU to run
+on, calls raise instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libptmf.so")
+
+PTMF_OK = 0
+ERRORS = {
+    1: "PTMF_ERR_ARG",
+    2: "PTMF_ERR_NOT_SETUP",
+    3: "PTMF_ERR_DEVICE",
+    4: "PTMF_ERR_NOMEM",
+    5: "PTMF_ERR_UNSUPPORTED",
+    6: "PTMF_ERR_DIVERGED",
+}
+
+# exported symbol -> (restype, argtypes); every one is declared in include/ptmf.h
+_P = C.c_void_p
+_I = C.c_int
+_L = C.c_long
+_D = C.c_double
+_PD = C.POINTER(C.c_double)
+SIGNATURES: dict[str, tuple] = {
+    "ptmf_last_error": (C.c_char_p, []),
+    "ptmf_version": (_I, [C.POINTER(_I), C.POINTER(_I)]),
+    "ptmf_device_count": (_I, [C.POINTER(_I)]),
+    "ptmf_set_device": (_I, [_I]),
+    "ptmf_malloc": (_I, [C.POINTER(_P), C.c_size_t]),
+    "ptmf_free": (_I, [_P]),
+    "ptmf_memcpy_htod": (_I, [_P, _P, C.c_size_t]),
+    "ptmf_memcpy_dtoh": (_I, [_P, _P, C.c_size_t]),
+    "ptmf_memset": (_I, [_P, _I, C.c_size_t]),
+    "ptmf_device_synchronize": (_I, []),
+    "ptmf_op_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
+    "ptmf_op_destroy": (_I, [C.POINTER(_P)]),
+    "ptmf_op_sizes": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
+    "ptmf_op_set_coordinates": (_I, [_P, _P]),
+    "ptmf_op_set_viscosity": (_I, [_P, _P]),
+    "ptmf_op_set_dirichlet": (_I, [_P, _P]),
+    "ptmf_op_set_coordinates_device": (_I, [_P, _P]),
+    "ptmf_op_set_viscosity_device": (_I, [_P, _P]),
+    "ptmf_op_state": (_I, [_P, C.POINTER(C.c_ulong)]),
+    "ptmf_a11_spmv": (_I, [_P, _P, _P]),
+    "ptmf_a11_spmv_host": (_I, [_P, _P, _P]),
+    "ptmf_a11_mult": (_I, [_P, _P, _P]),
+    "ptmf_a11_mult_f32": (_I, [_P, _P, _P]),
+    "ptmf_a11_diagonal": (_I, [_P, _P]),
+    "ptmf_a12_mult": (_I, [_P, _P, _P]),
+    "ptmf_a21_mult": (_I, [_P, _P, _P]),
+    "ptmf_a11_time": (_I, [_P, _I, _P, _P, _I, _PD]),
+    "ptmf_a11_time_kernel": (_I, [_P, _I, _P, _P, _I, _PD]),
+}
+
+
+class PtmfError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where}: {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libptmf.so (built by ptatin3d_amd.build); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(
+            f"{LIBPATH} is missing: build it with `python -m ptatin3d_amd.build` "
+            "(there is no fallback implementation)"
+        )
+    lib = C.CDLL(LIBPATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int, where: str) -> None:
+    if code != PTMF_OK:
+        msg = load().ptmf_last_error().decode(errors="replace")
+        raise PtmfError(code, where, msg)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args), name)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    call("ptmf_device_count", C.byref(n))
+    return n.value
+
+
+class DeviceArray:
+    """A device allocation made through the library (ptmf_malloc)."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        call("ptmf_malloc", C.byref(p), C.c_size_t(self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, dtype=None) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        call("ptmf_memcpy_htod", self.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        call("ptmf_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(self.nbytes))
+        return out
+
+    def zero(self) -> None:
+        call("ptmf_memset", self.ptr, 0, C.c_size_t(self.nbytes))
+
+    def free(self) -> None:
+        if self.ptr is not None and self.ptr.value:
+            call("ptmf_free", self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr is not None and self.ptr.value and _lib is not None:
+                _lib.ptmf_free(self.ptr)
+        except Exception:
+            pass
+
+
+def _hp(a: np.ndarray) -> C.c_void_p:
+    return a.ctypes.data_as(C.c_void_p)

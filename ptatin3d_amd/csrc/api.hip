@@ -1,0 +1,366 @@
+// api.hip -- C-ABI entry points of the ptmf operator library (see include/ptmf.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "ptmf_internal.h"
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+int ptmf_set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+using ptmf::Box;
+
+extern "C" {
+
+const char *ptmf_last_error(void) { return g_err; }
+
+int ptmf_version(int *major, int *minor) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  return PTMF_OK;
+}
+
+int ptmf_device_count(int *n) {
+  PTMF_CHECK(n, PTMF_ERR_ARG, "device_count: null");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *n = c;
+  return PTMF_OK;
+}
+
+int ptmf_set_device(int dev) {
+  PTMF_HIP(hipSetDevice(dev));
+  return PTMF_OK;
+}
+
+int ptmf_malloc(void **dptr, size_t bytes) {
+  PTMF_CHECK(dptr, PTMF_ERR_ARG, "malloc: null");
+  *dptr = nullptr;
+  if (bytes == 0) return PTMF_OK;
+  hipError_t e = hipMalloc(dptr, bytes);
+  if (e != hipSuccess) {
+    *dptr = nullptr;
+    return ptmf_set_error(PTMF_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  return PTMF_OK;
+}
+
+int ptmf_free(void *dptr) {
+  if (dptr) PTMF_HIP(hipFree(dptr));
+  return PTMF_OK;
+}
+
+int ptmf_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  if (bytes) PTMF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return PTMF_OK;
+}
+
+int ptmf_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  if (bytes) PTMF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return PTMF_OK;
+}
+
+int ptmf_memset(void *dptr, int value, size_t bytes) {
+  if (bytes) PTMF_HIP(hipMemset(dptr, value, bytes));
+  return PTMF_OK;
+}
+
+int ptmf_device_synchronize(void) {
+  PTMF_HIP(hipDeviceSynchronize());
+  return PTMF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// operator lifecycle
+// ---------------------------------------------------------------------------
+int ptmf_op_create(int mx, int my, int mz, ptmf_op *out) {
+  PTMF_CHECK(out, PTMF_ERR_ARG, "op_create: null handle");
+  *out = nullptr;
+  PTMF_CHECK(mx > 0 && my > 0 && mz > 0, PTMF_ERR_ARG, "op_create: bad box %d x %d x %d", mx, my, mz);
+  ptmf_op_s *op = new ptmf_op_s();
+  op->box = ptmf::make_box(mx, my, mz);
+  const Box &b = op->box;
+  int st = PTMF_OK;
+  auto alloc = [&](void **p, size_t n) {
+    if (st == PTMF_OK) st = ptmf_malloc(p, n);
+  };
+  alloc((void **)&op->d64.coords, sizeof(double) * 3 * b.nnodes);
+  alloc((void **)&op->d64.eta, sizeof(double) * 27 * b.nel);
+  alloc((void **)&op->d32.coords, sizeof(float) * 3 * b.nnodes);
+  alloc((void **)&op->d32.eta, sizeof(float) * 27 * b.nel);
+  alloc((void **)&op->bcnode, sizeof(uint8_t) * b.nnodes);
+  if (st == PTMF_OK) {
+    hipError_t e = hipStreamCreate(&op->stream);
+    if (e == hipSuccess) e = hipEventCreate(&op->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&op->ev1);
+    if (e != hipSuccess) st = ptmf_set_error(PTMF_ERR_DEVICE, "op_create: %s", hipGetErrorString(e));
+  }
+  if (st == PTMF_OK) st = ptmf_memset(op->bcnode, 0, b.nnodes);
+  if (st != PTMF_OK) {
+    ptmf_op tmp = op;
+    ptmf_op_destroy(&tmp);
+    return st;
+  }
+  *out = op;
+  return PTMF_OK;
+}
+
+int ptmf_op_destroy(ptmf_op *pop) {
+  if (!pop || !*pop) return PTMF_OK;
+  ptmf_op_s *op = *pop;
+  if (op->stream) (void)hipStreamSynchronize(op->stream);
+  (void)hipFree(op->d64.coords);
+  (void)hipFree(op->d64.eta);
+  (void)hipFree(op->d32.coords);
+  (void)hipFree(op->d32.eta);
+  (void)hipFree(op->bcnode);
+  (void)hipFree(op->bcidx);
+  (void)hipFree(op->host_stage);
+  (void)hipFree(op->host_stage2);
+  if (op->ev0) (void)hipEventDestroy(op->ev0);
+  if (op->ev1) (void)hipEventDestroy(op->ev1);
+  if (op->stream) (void)hipStreamDestroy(op->stream);
+  delete op;
+  *pop = nullptr;
+  return PTMF_OK;
+}
+
+int ptmf_op_sizes(ptmf_op op, long *nnodes, long *nel) {
+  PTMF_CHECK(op, PTMF_ERR_ARG, "op_sizes: null op");
+  if (nnodes) *nnodes = op->box.nnodes;
+  if (nel) *nel = op->box.nel;
+  return PTMF_OK;
+}
+
+int ptmf_op_state(ptmf_op op, unsigned long *state) {
+  PTMF_CHECK(op && state, PTMF_ERR_ARG, "op_state: null");
+  *state = op->state;
+  return PTMF_OK;
+}
+
+int ptmf_op_set_coordinates(ptmf_op op, const double *coords) {
+  PTMF_CHECK(op && coords, PTMF_ERR_ARG, "set_coordinates: null");
+  const long n = 3 * op->box.nnodes;
+  std::vector<float> c32(n);
+  for (long i = 0; i < n; i++) c32[i] = (float)coords[i];
+  PTMF_HIP(hipMemcpy(op->d64.coords, coords, sizeof(double) * n, hipMemcpyHostToDevice));
+  PTMF_HIP(hipMemcpy(op->d32.coords, c32.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  op->has_coords = 1;
+  op->state++;
+  return PTMF_OK;
+}
+
+int ptmf_op_set_coordinates_device(ptmf_op op, const double *coords) {
+  PTMF_CHECK(op && coords, PTMF_ERR_ARG, "set_coordinates_device: null");
+  const long n = 3 * op->box.nnodes;
+  std::vector<double> h(n);
+  PTMF_HIP(hipMemcpy(h.data(), coords, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return ptmf_op_set_coordinates(op, h.data());
+}
+
+// viscosity arrives element-major (eta[27e+q]) and is stored qp-major
+// (eta[q*nel+e]) so the lanes of a wave read consecutive addresses.
+int ptmf_op_set_viscosity(ptmf_op op, const double *eta) {
+  PTMF_CHECK(op && eta, PTMF_ERR_ARG, "set_viscosity: null");
+  const long nel = op->box.nel;
+  std::vector<double> t64(27 * nel);
+  std::vector<float> t32(27 * nel);
+  for (long e = 0; e < nel; e++)
+    for (int q = 0; q < 27; q++) {
+      const double v = eta[27 * e + q];
+      t64[(long)q * nel + e] = v;
+      t32[(long)q * nel + e] = (float)v;
+    }
+  PTMF_HIP(hipMemcpy(op->d64.eta, t64.data(), sizeof(double) * 27 * nel, hipMemcpyHostToDevice));
+  PTMF_HIP(hipMemcpy(op->d32.eta, t32.data(), sizeof(float) * 27 * nel, hipMemcpyHostToDevice));
+  op->has_eta = 1;
+  op->state++;
+  return PTMF_OK;
+}
+
+int ptmf_op_set_viscosity_device(ptmf_op op, const double *eta) {
+  PTMF_CHECK(op && eta, PTMF_ERR_ARG, "set_viscosity_device: null");
+  const long n = 27 * op->box.nel;
+  std::vector<double> h(n);
+  PTMF_HIP(hipMemcpy(h.data(), eta, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return ptmf_op_set_viscosity(op, h.data());
+}
+
+int ptmf_op_set_dirichlet(ptmf_op op, const unsigned char *mask) {
+  PTMF_CHECK(op && mask, PTMF_ERR_ARG, "set_dirichlet: null");
+  const Box &b = op->box;
+  std::vector<uint8_t> bits(b.nnodes, 0);
+  std::vector<long> idx;
+  for (long n = 0; n < b.nnodes; n++)
+    for (int d = 0; d < 3; d++)
+      if (mask[3 * n + d]) {
+        bits[n] |= (uint8_t)(1u << d);
+        idx.push_back(3 * n + d);
+      }
+  PTMF_HIP(hipMemcpy(op->bcnode, bits.data(), b.nnodes, hipMemcpyHostToDevice));
+  if (op->bcidx) {
+    PTMF_HIP(hipFree(op->bcidx));
+    op->bcidx = nullptr;
+  }
+  op->nbc = (long)idx.size();
+  if (op->nbc) {
+    PTMF_TRY(ptmf_malloc((void **)&op->bcidx, sizeof(long) * op->nbc));
+    PTMF_HIP(hipMemcpy(op->bcidx, idx.data(), sizeof(long) * op->nbc, hipMemcpyHostToDevice));
+  }
+  op->has_bc = 1;
+  op->state++;
+  return PTMF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// applies
+// ---------------------------------------------------------------------------
+namespace {
+
+int check_ready(const ptmf_op_s *op, bool need_eta, bool need_bc) {
+  PTMF_CHECK(op, PTMF_ERR_ARG, "null operator");
+  PTMF_CHECK(op->has_coords && (!need_eta || op->has_eta) && (!need_bc || op->has_bc),
+             PTMF_ERR_NOT_SETUP, "MF operator not setup");
+  return PTMF_OK;
+}
+
+template <typename T>
+int a11_mult(ptmf_op_s *op, const T *x, T *y) {
+  PTMF_TRY(check_ready(op, true, true));
+  PTMF_CHECK(x && y && (const void *)x != (const void *)y, PTMF_ERR_ARG,
+             "a11_mult: x and y must be distinct device arrays");
+  const long n3 = 3 * op->box.nnodes;
+  PTMF_TRY(ptmf::launch_fill<T>(y, T(0), n3, op->stream));
+  PTMF_TRY(ptmf::launch_a11<T>(op, x, y, true));
+  PTMF_TRY(ptmf::launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
+  return PTMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ptmf_a11_spmv(ptmf_op op, const double *u, double *y) {
+  PTMF_TRY(check_ready(op, true, false));
+  PTMF_CHECK(u && y && u != y, PTMF_ERR_ARG, "a11_spmv: u and y must be distinct device arrays");
+  PTMF_TRY(ptmf::launch_a11<double>(op, u, y, false));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
+int ptmf_a11_spmv_host(ptmf_op op, const double *u, double *Yu) {
+  PTMF_TRY(check_ready(op, true, false));
+  PTMF_CHECK(u && Yu, PTMF_ERR_ARG, "a11_spmv_host: null");
+  const long n3 = 3 * op->box.nnodes;
+  if (!op->host_stage) PTMF_TRY(ptmf_malloc((void **)&op->host_stage, sizeof(double) * n3));
+  if (!op->host_stage2) PTMF_TRY(ptmf_malloc((void **)&op->host_stage2, sizeof(double) * n3));
+  PTMF_HIP(hipMemcpyAsync(op->host_stage, u, sizeof(double) * n3, hipMemcpyHostToDevice, op->stream));
+  PTMF_HIP(hipMemcpyAsync(op->host_stage2, Yu, sizeof(double) * n3, hipMemcpyHostToDevice, op->stream));
+  PTMF_TRY(ptmf::launch_a11<double>(op, op->host_stage, op->host_stage2, false));
+  PTMF_HIP(hipMemcpyAsync(Yu, op->host_stage2, sizeof(double) * n3, hipMemcpyDeviceToHost, op->stream));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
+int ptmf_a11_mult(ptmf_op op, const double *x, double *y) {
+  PTMF_TRY(a11_mult<double>(op, x, y));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
+int ptmf_a11_mult_f32(ptmf_op op, const float *x, float *y) {
+  PTMF_TRY(a11_mult<float>(op, x, y));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
+int ptmf_a11_diagonal(ptmf_op op, double *d) {
+  PTMF_TRY(check_ready(op, true, true));
+  PTMF_CHECK(d, PTMF_ERR_ARG, "a11_diagonal: null");
+  PTMF_TRY(ptmf::launch_fill<double>(d, 0.0, 3 * op->box.nnodes, op->stream));
+  PTMF_TRY(ptmf::launch_a11_diag(op, d));
+  PTMF_TRY(ptmf::launch_bc_set<double>(op->bcidx, op->nbc, 1.0, d, op->stream));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
+int ptmf_a12_mult(ptmf_op op, const double *p, double *yu) {
+  PTMF_TRY(check_ready(op, false, true));
+  PTMF_CHECK(p && yu, PTMF_ERR_ARG, "a12_mult: null");
+  PTMF_TRY(ptmf::launch_fill<double>(yu, 0.0, 3 * op->box.nnodes, op->stream));
+  PTMF_TRY(ptmf::launch_a12(op, p, yu));
+  PTMF_TRY(ptmf::launch_bc_set<double>(op->bcidx, op->nbc, 0.0, yu, op->stream));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
+int ptmf_a21_mult(ptmf_op op, const double *u, double *yp) {
+  PTMF_TRY(check_ready(op, false, true));
+  PTMF_CHECK(u && yp, PTMF_ERR_ARG, "a21_mult: null");
+  PTMF_TRY(ptmf::launch_a21(op, u, yp));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <typename F>
+int time_loop(ptmf_op_s *op, int reps, F body, double *ms_out) {
+  PTMF_HIP(hipEventRecord(op->ev0, op->stream));
+  for (int r = 0; r < reps; r++) PTMF_TRY(body());
+  PTMF_HIP(hipEventRecord(op->ev1, op->stream));
+  PTMF_HIP(hipEventSynchronize(op->ev1));
+  float ms = 0.f;
+  PTMF_HIP(hipEventElapsedTime(&ms, op->ev0, op->ev1));
+  if (ms_out) *ms_out = ms / reps;
+  return PTMF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ptmf_a11_time(ptmf_op op, int precision, const void *x, void *y, int reps,
+                  double *ms_per_mult) {
+  PTMF_CHECK(reps > 0, PTMF_ERR_ARG, "a11_time: reps must be > 0");
+  PTMF_CHECK(precision == 64 || precision == 32, PTMF_ERR_UNSUPPORTED,
+             "a11_time: precision %d (64 or 32)", precision);
+  PTMF_TRY(check_ready(op, true, true));
+  if (precision == 64)
+    return time_loop(op, reps, [&] { return a11_mult<double>(op, (const double *)x, (double *)y); },
+                     ms_per_mult);
+  return time_loop(op, reps, [&] { return a11_mult<float>(op, (const float *)x, (float *)y); },
+                   ms_per_mult);
+}
+
+int ptmf_a11_time_kernel(ptmf_op op, int precision, const void *x, void *y, int reps,
+                         double *ms_per_kernel) {
+  PTMF_CHECK(reps > 0, PTMF_ERR_ARG, "a11_time_kernel: reps must be > 0");
+  PTMF_CHECK(precision == 64 || precision == 32, PTMF_ERR_UNSUPPORTED,
+             "a11_time_kernel: precision %d (64 or 32)", precision);
+  PTMF_TRY(check_ready(op, true, true));
+  if (precision == 64)
+    return time_loop(
+        op, reps, [&] { return ptmf::launch_a11<double>(op, (const double *)x, (double *)y, true); },
+        ms_per_kernel);
+  return time_loop(
+      op, reps, [&] { return ptmf::launch_a11<float>(op, (const float *)x, (float *)y, true); },
+      ms_per_kernel);
+}
+
+}  // extern "C"

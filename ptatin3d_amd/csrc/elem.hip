@@ -1,0 +1,266 @@
+// elem.hip -- per-element Q2 kernels other than the A11 apply: the A11
+// diagonal and the pressure-velocity couplings B (A12) and B^T (A21).
+//
+// Layout of a work-group: 64 lanes, 2 elements, lanes 32h + q (q < 27) own
+// quadrature point q of element h, lanes 32h + n (n < 27) own node n in the
+// node phase.  The element's 27 node coordinates are staged in LDS once and
+// read back as broadcasts; per-qp geometry (inverse Jacobian, weights) goes
+// through LDS from the qp phase to the node phase.
+#include "ptmf_internal.h"
+
+namespace ptmf {
+namespace {
+
+__constant__ double cB[3][3];
+__constant__ double cD[3][3];
+__constant__ double cW[3];
+__constant__ int cIsInit;
+
+constexpr int kEPB = 2;  // elements per 64-lane work-group
+
+struct QpGeom {
+  double G[3][3];  // G[k][j] = d xi_j / d x_k
+  double wdet;     // w_q * detJ
+  double xq[3];    // physical position of the qp
+};
+
+// Geometry at quadrature point q = (qa,qb,qc) from the element's node
+// coordinates xe[n][d] (P3D_evaluate_geometry_elementQ2,
+// element_utils_q2.c:400-476, with the exact 3-point Gauss rule).
+__device__ void qp_geometry(const double (*xe)[3], int q, QpGeom &g) {
+  const int qa = q % 3, qb = (q / 3) % 3, qc = q / 9;
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double xq[3] = {0, 0, 0};
+  for (int k = 0; k < 3; k++)
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++) {
+        const int n = i + 3 * j + 9 * k;
+        const double Ni = cB[qa][i], Nj = cB[qb][j], Nk = cB[qc][k];
+        const double d0 = cD[qa][i] * Nj * Nk, d1 = Ni * cD[qb][j] * Nk, d2 = Ni * Nj * cD[qc][k];
+        const double N = Ni * Nj * Nk;
+        for (int d = 0; d < 3; d++) {
+          const double x = xe[n][d];
+          J[0][d] += d0 * x;
+          J[1][d] += d1 * x;
+          J[2][d] += d2 * x;
+          xq[d] += N * x;
+        }
+      }
+  double A[3][3];
+  A[0][0] = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+  A[0][1] = J[2][1] * J[0][2] - J[0][1] * J[2][2];
+  A[0][2] = J[0][1] * J[1][2] - J[1][1] * J[0][2];
+  A[1][0] = J[2][0] * J[1][2] - J[1][0] * J[2][2];
+  A[1][1] = J[0][0] * J[2][2] - J[2][0] * J[0][2];
+  A[1][2] = J[1][0] * J[0][2] - J[0][0] * J[1][2];
+  A[2][0] = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+  A[2][1] = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+  A[2][2] = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+  const double det = J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
+  const double id = 1.0 / det;
+  for (int r = 0; r < 3; r++)
+    for (int s = 0; s < 3; s++) g.G[r][s] = A[r][s] * id;
+  g.wdet = cW[qa] * cW[qb] * cW[qc] * det;
+  for (int d = 0; d < 3; d++) g.xq[d] = xq[d];
+}
+
+// reference-coordinate gradient of node n's basis function at qp q
+__device__ __forceinline__ void grad_ref(int n, int q, double gr[3]) {
+  const int i = n % 3, j = (n / 3) % 3, k = n / 9;
+  const int qa = q % 3, qb = (q / 3) % 3, qc = q / 9;
+  const double Ni = cB[qa][i], Nj = cB[qb][j], Nk = cB[qc][k];
+  gr[0] = cD[qa][i] * Nj * Nk;
+  gr[1] = Ni * cD[qb][j] * Nk;
+  gr[2] = Ni * Nj * cD[qc][k];
+}
+
+__device__ __forceinline__ long elem_node(int mx, int my, long X, long XY, long e, int n) {
+  const long ei = e % mx, er = e / mx, ej = er % my, ek = er / my;
+  const int i = n % 3, j = (n / 3) % 3, k = n / 9;
+  return (2 * ei + i) + X * (2 * ej + j) + XY * (2 * ek + k);
+}
+
+// P1disc basis relative to the element (element_utils_q2.c:183-224)
+__device__ __forceinline__ void p1_basis(const double (*xe)[3], const double xq[3], double Np[4]) {
+  double avg[3] = {0, 0, 0};
+  for (int n = 0; n < 27; n++)
+    for (int d = 0; d < 3; d++) avg[d] += xe[n][d];
+  const double L[3] = {xe[14][0] - xe[12][0], xe[16][1] - xe[10][1], xe[22][2] - xe[4][2]};
+  Np[0] = 1.0;
+  for (int d = 0; d < 3; d++) Np[1 + d] = (xq[d] - (1.0 / 27.0) * avg[d]) / L[d];
+}
+
+// ---- diag(A11): d[3n+c] += sum_q w detJ eta (|grad N_n|^2 + (d_c N_n)^2)
+// (MFStokesWrapper_diagA11, stokes_operators_mf.c:67-131 with
+// MatMultMF_Stokes_MixedFEM3d_diagB11, stokes_q2p1_mf_operators_diag_def.c:120-205)
+__global__ void __launch_bounds__(64)
+    a11_diag_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
+                    const double *__restrict__ eta, double *__restrict__ dg) {
+  __shared__ double sx[kEPB][27][3];
+  __shared__ double sG[kEPB][27][10];
+  const int h = threadIdx.x >> 5, t = threadIdx.x & 31;
+  long e = (long)blockIdx.x * kEPB + h;
+  const bool live = e < nel;
+  if (!live) e = nel - 1;
+  if (t < 27) {
+    const long node = elem_node(mx, my, X, XY, e, t);
+    for (int d = 0; d < 3; d++) sx[h][t][d] = xc[3 * node + d];
+  }
+  __syncthreads();
+  if (t < 27) {
+    QpGeom g;
+    qp_geometry(sx[h], t, g);
+    for (int r = 0; r < 3; r++)
+      for (int s = 0; s < 3; s++) sG[h][t][3 * r + s] = g.G[r][s];
+    sG[h][t][9] = g.wdet * eta[(long)t * nel + e];
+  }
+  __syncthreads();
+  if (t < 27 && live) {
+    double acc[3] = {0, 0, 0};
+    for (int q = 0; q < 27; q++) {
+      double gr[3], gp[3];
+      grad_ref(t, q, gr);
+      const double *G = sG[h][q];
+      for (int k = 0; k < 3; k++) gp[k] = G[3 * k] * gr[0] + G[3 * k + 1] * gr[1] + G[3 * k + 2] * gr[2];
+      const double s = G[9];
+      const double g2 = gp[0] * gp[0] + gp[1] * gp[1] + gp[2] * gp[2];
+      for (int c = 0; c < 3; c++) acc[c] += s * (g2 + gp[c] * gp[c]);
+    }
+    const long node = elem_node(mx, my, X, XY, e, t);
+    for (int c = 0; c < 3; c++) atomicAdd(&dg[3 * node + c], acc[c]);
+  }
+}
+
+// ---- B p: yu[3n+c] += sum_q -w detJ (d_c N_n) p_h(x_q)
+// (MFStokesWrapper_A12, stokes_operators_mf.c:788-856 with
+// MatMultMF_Stokes_MixedFEM3d_A12, stokes_q2p1_mf_operators_def.c:423-516)
+__global__ void __launch_bounds__(64)
+    a12_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
+               const double *__restrict__ p, double *__restrict__ yu) {
+  __shared__ double sx[kEPB][27][3];
+  __shared__ double sG[kEPB][27][10];
+  const int h = threadIdx.x >> 5, t = threadIdx.x & 31;
+  long e = (long)blockIdx.x * kEPB + h;
+  const bool live = e < nel;
+  if (!live) e = nel - 1;
+  if (t < 27) {
+    const long node = elem_node(mx, my, X, XY, e, t);
+    for (int d = 0; d < 3; d++) sx[h][t][d] = xc[3 * node + d];
+  }
+  __syncthreads();
+  if (t < 27) {
+    QpGeom g;
+    qp_geometry(sx[h], t, g);
+    double Np[4];
+    p1_basis(sx[h], g.xq, Np);
+    const double ph = Np[0] * p[4 * e] + Np[1] * p[4 * e + 1] + Np[2] * p[4 * e + 2] +
+                      Np[3] * p[4 * e + 3];
+    for (int r = 0; r < 3; r++)
+      for (int s = 0; s < 3; s++) sG[h][t][3 * r + s] = g.G[r][s];
+    sG[h][t][9] = -g.wdet * ph;
+  }
+  __syncthreads();
+  if (t < 27 && live) {
+    double acc[3] = {0, 0, 0};
+    for (int q = 0; q < 27; q++) {
+      double gr[3];
+      grad_ref(t, q, gr);
+      const double *G = sG[h][q];
+      for (int k = 0; k < 3; k++)
+        acc[k] += G[9] * (G[3 * k] * gr[0] + G[3 * k + 1] * gr[1] + G[3 * k + 2] * gr[2]);
+    }
+    const long node = elem_node(mx, my, X, XY, e, t);
+    for (int c = 0; c < 3; c++) atomicAdd(&yu[3 * node + c], acc[c]);
+  }
+}
+
+// ---- B^T u: yp[4e+i] = sum_q -w detJ NIp_i div u_h(x_q), u Dirichlet-masked
+// (MFStokesWrapper_A21, stokes_operators_mf.c:858-933 with
+// MatMultMF_Stokes_MixedFEM3d_A21, stokes_q2p1_mf_operators_def.c:517-538)
+__global__ void __launch_bounds__(64)
+    a21_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
+               const uint8_t *__restrict__ bcn, const double *__restrict__ u,
+               double *__restrict__ yp) {
+  __shared__ double sx[kEPB][27][3];
+  __shared__ double su[kEPB][27][3];
+  __shared__ double sr[kEPB][32][4];
+  const int h = threadIdx.x >> 5, t = threadIdx.x & 31;
+  long e = (long)blockIdx.x * kEPB + h;
+  const bool live = e < nel;
+  if (!live) e = nel - 1;
+  if (t < 27) {
+    const long node = elem_node(mx, my, X, XY, e, t);
+    const unsigned m = bcn[node];
+    for (int d = 0; d < 3; d++) {
+      sx[h][t][d] = xc[3 * node + d];
+      su[h][t][d] = ((m >> d) & 1u) ? 0.0 : u[3 * node + d];
+    }
+  }
+  __syncthreads();
+  double val[4] = {0, 0, 0, 0};
+  if (t < 27) {
+    QpGeom g;
+    qp_geometry(sx[h], t, g);
+    double Np[4];
+    p1_basis(sx[h], g.xq, Np);
+    double div = 0.0;
+    for (int n = 0; n < 27; n++) {
+      double gr[3];
+      grad_ref(n, t, gr);
+      for (int k = 0; k < 3; k++)
+        div += su[h][n][k] * (g.G[k][0] * gr[0] + g.G[k][1] * gr[1] + g.G[k][2] * gr[2]);
+    }
+    for (int i = 0; i < 4; i++) val[i] = -g.wdet * Np[i] * div;
+  }
+  for (int i = 0; i < 4; i++) sr[h][t][i] = val[i];
+  __syncthreads();
+  if (t < 4 && live) {
+    double s = 0.0;
+    for (int q = 0; q < 27; q++) s += sr[h][q][t];
+    yp[4 * e + t] = s;
+  }
+}
+
+int init_constants() {
+  static bool done = false;
+  if (done) return PTMF_OK;
+  PTMF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cB), kB, sizeof(kB)));
+  PTMF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cD), kD, sizeof(kD)));
+  PTMF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cW), kW, sizeof(kW)));
+  done = true;
+  return PTMF_OK;
+}
+
+}  // namespace
+
+int launch_a11_diag(const ptmf_op_s *op, double *d) {
+  PTMF_TRY(init_constants());
+  const Box &b = op->box;
+  const long nblk = (b.nel + kEPB - 1) / kEPB;
+  a11_diag_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
+      b.mx, b.my, b.nel, b.X, b.X * b.Y, op->d64.coords, op->d64.eta, d);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+int launch_a12(const ptmf_op_s *op, const double *p, double *yu) {
+  PTMF_TRY(init_constants());
+  const Box &b = op->box;
+  const long nblk = (b.nel + kEPB - 1) / kEPB;
+  a12_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(b.mx, b.my, b.nel, b.X,
+                                                                 b.X * b.Y, op->d64.coords, p, yu);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+int launch_a21(const ptmf_op_s *op, const double *u, double *yp) {
+  PTMF_TRY(init_constants());
+  const Box &b = op->box;
+  const long nblk = (b.nel + kEPB - 1) / kEPB;
+  a21_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
+      b.mx, b.my, b.nel, b.X, b.X * b.Y, op->d64.coords, op->bcnode, u, yp);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+}  // namespace ptmf

@@ -1,0 +1,113 @@
+// ptmf_internal.h -- shared definitions of the ptmf HIP library (gfx950).
+//
+// Mesh model: one structured box of mx*my*mz Q2 elements, X=2mx+1, Y=2my+1,
+// Z=2mz+1 nodes, node (i,j,k) -> i + X*(j + Y*k) (the DMDA numbering of
+// dmda_element_q2p1.c:395-440).  Velocity vectors are node-interlaced
+// (dof 3*node+d), as PETSc local vectors of the velocity DMDA.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "ptmf.h"
+
+namespace ptmf {
+
+// ---------------------------------------------------------------------------
+// 1-D tables at the 3 Gauss-Legendre points g = (-s, 0, s), s = sqrt(3/5).
+// B[q][n]: quadratic Lagrange basis (nodes -1,0,1) at g_q; D[q][n]: its
+// derivative (stokes_operators_tensor.c:195-202).  C[q][r]: derivative of the
+// Lagrange basis on the Gauss points themselves, L'_r(g_q) (collocation
+// derivative, C = D B^{-1}); it lets the kernels interpolate once to the
+// quadrature grid and differentiate there.
+// ---------------------------------------------------------------------------
+constexpr double kS = 0.77459666924148337703585307995647992;  // sqrt(3/5)
+constexpr double kB[3][3] = {{0.3 + 0.5 * kS, 0.4, 0.3 - 0.5 * kS},
+                             {0.0, 1.0, 0.0},
+                             {0.3 - 0.5 * kS, 0.4, 0.3 + 0.5 * kS}};
+constexpr double kD[3][3] = {{-kS - 0.5, 2.0 * kS, 0.5 - kS},
+                             {-0.5, 0.0, 0.5},
+                             {kS - 0.5, -2.0 * kS, kS + 0.5}};
+constexpr double kC[3][3] = {{-2.5 * kS, (10.0 / 3.0) * kS, -(5.0 / 6.0) * kS},
+                             {-(5.0 / 6.0) * kS, 0.0, (5.0 / 6.0) * kS},
+                             {(5.0 / 6.0) * kS, -(10.0 / 3.0) * kS, 2.5 * kS}};
+constexpr double kW[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+
+struct Box {
+  int mx, my, mz;
+  long X, Y, Z;
+  long nel, nnodes;
+};
+
+inline Box make_box(int mx, int my, int mz) {
+  Box b;
+  b.mx = mx; b.my = my; b.mz = mz;
+  b.X = 2L * mx + 1; b.Y = 2L * my + 1; b.Z = 2L * mz + 1;
+  b.nel = (long)mx * my * mz;
+  b.nnodes = b.X * b.Y * b.Z;
+  return b;
+}
+
+// Per-precision device copies of the operator data.
+template <typename T>
+struct OpData {
+  T *coords = nullptr;   // 3N node-interlaced coordinates
+  T *eta = nullptr;      // 27*nel, quadrature-point-major: eta[q*nel + e]
+};
+
+}  // namespace ptmf
+
+// Operator handle (opaque in the C-ABI).
+struct ptmf_op_s {
+  ptmf::Box box;
+  ptmf::OpData<double> d64;
+  ptmf::OpData<float> d32;
+  uint8_t *bcnode = nullptr;      // per-node Dirichlet bits (bit d: dof d)
+  long *bcidx = nullptr;          // compact list of Dirichlet dofs
+  long nbc = 0;
+  int has_coords = 0, has_eta = 0, has_bc = 0;
+  unsigned long state = 0;        // bumped by every coefficient change
+  hipStream_t stream = nullptr;   // stream all work of this operator runs on
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double *host_stage = nullptr;   // device staging for the host-array entry points
+  double *host_stage2 = nullptr;
+};
+
+// Error plumbing: every entry point returns a ptmf status code.
+int ptmf_set_error(int code, const char *fmt, ...);
+
+#define PTMF_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return ptmf_set_error(PTMF_ERR_DEVICE, "%s:%d %s: %s", __FILE__, __LINE__,   \
+                            #call, hipGetErrorString(e_));                          \
+  } while (0)
+
+#define PTMF_CHECK(cond, code, ...)                      \
+  do {                                                   \
+    if (!(cond)) return ptmf_set_error(code, __VA_ARGS__); \
+  } while (0)
+
+#define PTMF_TRY(call)            \
+  do {                            \
+    int s_ = (call);              \
+    if (s_ != PTMF_OK) return s_; \
+  } while (0)
+
+// Kernel launch helpers implemented in the .hip files (all asynchronous on
+// the operator's stream).
+namespace ptmf {
+template <typename T>
+int launch_a11(const ptmf_op_s *op, const T *u, T *y, bool mask_columns);
+template <typename T>
+int launch_fill(T *x, T v, long n, hipStream_t s);
+template <typename T>
+int launch_bc_copy(const long *idx, long n, const T *x, T *y, hipStream_t s);
+template <typename T>
+int launch_bc_set(const long *idx, long n, T v, T *y, hipStream_t s);
+int launch_a11_diag(const ptmf_op_s *op, double *d);
+int launch_a12(const ptmf_op_s *op, const double *p, double *yu);
+int launch_a21(const ptmf_op_s *op, const double *u, double *yp);
+}  // namespace ptmf

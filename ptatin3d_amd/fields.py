@@ -1,0 +1,147 @@
+"""Seeded synthetic meshes, viscosity fields and vectors for the Q2 box workloads.
+
+Everything is deterministic in (sizes, seed), so the GPU operator and the CPU
+oracle are fed identical arrays.  Layouts follow include/ptmf.h.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SEED = 20261202
+GAUSS = np.array([-np.sqrt(0.6), 0.0, np.sqrt(0.6)])
+
+
+def nnodes(mx: int, my: int, mz: int) -> int:
+    return (2 * mx + 1) * (2 * my + 1) * (2 * mz + 1)
+
+
+def box_coords(mx, my, mz, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """Affine box, node-interlaced coordinates (3N)."""
+    X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+    xs = np.linspace(lo[0], hi[0], X)
+    ys = np.linspace(lo[1], hi[1], Y)
+    zs = np.linspace(lo[2], hi[2], Z)
+    zz, yy, xx = np.meshgrid(zs, ys, xs, indexing="ij")
+    return np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1).reshape(-1).copy()
+
+
+def boundary_node_flags(mx, my, mz) -> np.ndarray:
+    """Boolean per node: lies on any face of the box."""
+    X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+    k, j, i = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    b = (i == 0) | (i == X - 1) | (j == 0) | (j == Y - 1) | (k == 0) | (k == Z - 1)
+    return b.ravel()
+
+
+def perturb_interior(coords, mx, my, mz, frac=0.1, rng=None) -> np.ndarray:
+    """Move every interior node by U(-frac*h, frac*h) per coordinate, h the
+    node spacing of each direction."""
+    rng = rng or np.random.default_rng(SEED)
+    c = coords.reshape(-1, 3).copy()
+    h = np.array([1.0 / (2 * mx), 1.0 / (2 * my), 1.0 / (2 * mz)])
+    h = h * (c.max(axis=0) - c.min(axis=0))
+    inner = ~boundary_node_flags(mx, my, mz)
+    c[inner] += rng.uniform(-1.0, 1.0, size=(inner.sum(), 3)) * (frac * h)
+    return c.reshape(-1)
+
+
+def dirichlet_all_faces(mx, my, mz) -> np.ndarray:
+    """All three velocity components fixed on all six faces (3N bytes)."""
+    b = boundary_node_flags(mx, my, mz)
+    return np.repeat(b, 3).astype(np.uint8)
+
+
+def element_node_ids(mx, my, mz) -> np.ndarray:
+    """[E, 27] node indices, element e = ei + mx*(ej + my*ek), local n = i+3j+9k."""
+    X, Y = 2 * mx + 1, 2 * my + 1
+    ek, ej, ei = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    ei, ej, ek = ei.ravel(), ej.ravel(), ek.ravel()
+    k, j, i = np.meshgrid(np.arange(3), np.arange(3), np.arange(3), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    return (2 * ei[:, None] + i[None]) + X * ((2 * ej[:, None] + j[None]) + Y * (2 * ek[:, None] + k[None]))
+
+
+def _q2_basis_at_qps() -> np.ndarray:
+    """[27 qp, 27 node] Q2 basis values at the 3x3x3 Gauss points."""
+    x = GAUSS
+    N1 = np.stack([0.5 * x * (x - 1.0), (1.0 + x) * (1.0 - x), 0.5 * (1.0 + x) * x], axis=1)  # [q, n]
+    out = np.einsum("ai,bj,ck->cbakji", N1, N1, N1).reshape(27, 27)
+    return out
+
+
+def qp_coordinates(coords, mx, my, mz) -> np.ndarray:
+    """[E, 27, 3] physical positions of the quadrature points."""
+    nid = element_node_ids(mx, my, mz)
+    xe = coords.reshape(-1, 3)[nid]  # [E, 27, 3]
+    return np.einsum("qn,end->eqd", _q2_basis_at_qps(), xe)
+
+
+def eta_loguniform(nel, lo=1e-2, hi=1e2, rng=None) -> np.ndarray:
+    rng = rng or np.random.default_rng(SEED + 1)
+    return 10.0 ** rng.uniform(np.log10(lo), np.log10(hi), size=27 * nel)
+
+
+def eta_sinker(qpx: np.ndarray, nspheres=32, rmin=0.05, rmax=0.12, eta_bg=1.0, eta_s=1e4,
+               rng=None) -> np.ndarray:
+    """Background eta_bg with eta_s inside seeded spheres, evaluated at qps."""
+    rng = rng or np.random.default_rng(SEED + 2)
+    cen = rng.uniform(0.0, 1.0, size=(nspheres, 3))
+    rad = rng.uniform(rmin, rmax, size=nspheres)
+    p = qpx.reshape(-1, 3)
+    eta = np.full(p.shape[0], eta_bg)
+    for c, r in zip(cen, rad):
+        inside = ((p - c) ** 2).sum(axis=1) <= r * r
+        eta[inside] = eta_s
+    return eta
+
+
+def eta_layered_ellipsoids(qpx: np.ndarray, n_ell=64, rng=None) -> np.ndarray:
+    """Three z-layers at 1e3 / 1e1 / 1e-3 (bottom to top) plus seeded weak
+    ellipsoids at 1e-3: viscosity contrast 1e6."""
+    rng = rng or np.random.default_rng(SEED + 3)
+    p = qpx.reshape(-1, 3)
+    z = p[:, 2]
+    eta = np.where(z < 1.0 / 3.0, 1e3, np.where(z < 2.0 / 3.0, 1e1, 1e-3))
+    cen = rng.uniform(0.0, 1.0, size=(n_ell, 3))
+    axes = rng.uniform(0.03, 0.1, size=(n_ell, 3))
+    for c, a in zip(cen, axes):
+        inside = (((p - c) / a) ** 2).sum(axis=1) <= 1.0
+        eta[inside] = 1e-3
+    return eta
+
+
+def random_velocity(mx, my, mz, bcmask=None, rng=None) -> np.ndarray:
+    rng = rng or np.random.default_rng(SEED + 4)
+    u = rng.uniform(-1.0, 1.0, size=3 * nnodes(mx, my, mz))
+    if bcmask is not None:
+        u[bcmask.astype(bool)] = 0.0
+    return u
+
+
+def reference_test_vector(coords) -> np.ndarray:
+    """f = x + y + z + component + 3 (test_stokes_operators.c:48-92)."""
+    c = coords.reshape(-1, 3)
+    s = c.sum(axis=1)
+    return np.stack([s + 3.0, s + 4.0, s + 5.0], axis=1).reshape(-1)
+
+
+# ---- named workloads -------------------------------------------------------
+
+def config1(m=16, seed=SEED):
+    """16^3 perturbed unit cube, log-uniform eta in [1e-2,1e2], random u."""
+    rng = np.random.default_rng(seed)
+    coords = perturb_interior(box_coords(m, m, m), m, m, m, 0.1, rng)
+    eta = eta_loguniform(m ** 3, 1e-2, 1e2, rng)
+    bc = dirichlet_all_faces(m, m, m)
+    u = random_velocity(m, m, m, bc, rng)
+    return dict(mx=m, my=m, mz=m, coords=coords, eta=eta, bc=bc, u=u)
+
+
+def config2(m=64, seed=SEED):
+    """64^3 affine unit cube, sinker viscosity (32 spheres at 1e4), random u."""
+    rng = np.random.default_rng(seed)
+    coords = box_coords(m, m, m)
+    eta = eta_sinker(qp_coordinates(coords, m, m, m), rng=rng)
+    bc = dirichlet_all_faces(m, m, m)
+    u = random_velocity(m, m, m, bc, rng)
+    return dict(mx=m, my=m, mz=m, coords=coords, eta=eta, bc=bc, u=u)

@@ -1,0 +1,160 @@
+"""Host-side mirror of pTatin3d's matrix-free operator objects over the ptmf C-ABI.
+
+`MatA11MF` plays the role of the reference's `struct _p_MatA11MF`
+(include/stokes_operators.h:47-62) with the `-a11_op hip` backend:
+create -> setup (coordinates, quadrature viscosity, Dirichlet list) ->
+MatMult / MatGetDiagonal.  `MatStokesMF` adds the B / B^T couplings
+(MatMult_MFStokes_A12 / _A21).  Arrays are numpy on the host or
+`DeviceArray` on the GPU; layouts are those of include/ptmf.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import DeviceArray, call
+
+A11_OPERATOR_TYPES = ("hip",)
+
+
+class MatA11MF:
+    """Matrix-free Q2 viscous block on one structured box, device resident."""
+
+    def __init__(self, mx: int, my: int, mz: int, a11_op: str = "hip"):
+        # MatA11MFCreate (stokes_operators.c:101-157): unknown backends are an error
+        if a11_op not in A11_OPERATOR_TYPES:
+            raise _lib.PtmfError(5, "MatA11MFCreate", f"No -a11_op {a11_op}")
+        self.mx, self.my, self.mz = int(mx), int(my), int(mz)
+        h = C.c_void_p()
+        call("ptmf_op_create", self.mx, self.my, self.mz, C.byref(h))
+        self.handle = h
+        self.nnodes = (2 * mx + 1) * (2 * my + 1) * (2 * mz + 1)
+        self.nel = mx * my * mz
+        self.is_setup = False
+
+    # -- MatA11MFSetup ----------------------------------------------------
+    def setup(self, coords: np.ndarray, eta: np.ndarray, bcmask: np.ndarray) -> "MatA11MF":
+        self.set_coordinates(coords)
+        self.set_viscosity(eta)
+        self.set_dirichlet(bcmask)
+        self.is_setup = True
+        return self
+
+    def set_coordinates(self, coords: np.ndarray) -> None:
+        c = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+        assert c.size == 3 * self.nnodes, (c.size, self.nnodes)
+        call("ptmf_op_set_coordinates", self.handle, _lib._hp(c))
+
+    def set_viscosity(self, eta: np.ndarray) -> None:
+        e = np.ascontiguousarray(eta, dtype=np.float64).reshape(-1)
+        assert e.size == 27 * self.nel
+        call("ptmf_op_set_viscosity", self.handle, _lib._hp(e))
+
+    def set_dirichlet(self, bcmask: np.ndarray) -> None:
+        m = np.ascontiguousarray(bcmask, dtype=np.uint8).reshape(-1)
+        assert m.size == 3 * self.nnodes
+        call("ptmf_op_set_dirichlet", self.handle, _lib._hp(m))
+
+    @property
+    def state(self) -> int:
+        s = C.c_ulong(0)
+        call("ptmf_op_state", self.handle, C.byref(s))
+        return s.value
+
+    # -- operations on device arrays --------------------------------------
+    def mult(self, x: DeviceArray, y: DeviceArray) -> None:
+        """MatMult_MFStokes_A11 (stokes_operators.c:1142-1205)."""
+        if x.dtype == np.float32:
+            call("ptmf_a11_mult_f32", self.handle, x.ptr, y.ptr)
+        else:
+            call("ptmf_a11_mult", self.handle, x.ptr, y.ptr)
+
+    def spmv(self, u: DeviceArray, y: DeviceArray) -> None:
+        """SpMVOp_MatMult: y += A11_local u (no boundary handling)."""
+        call("ptmf_a11_spmv", self.handle, u.ptr, y.ptr)
+
+    def get_diagonal(self, d: DeviceArray) -> None:
+        """MatGetDiagonal_MFStokes_A11 (stokes_operators.c:1060-1099)."""
+        call("ptmf_a11_diagonal", self.handle, d.ptr)
+
+    # -- host conveniences --------------------------------------------------
+    def mult_host(self, x: np.ndarray, precision: int = 64) -> np.ndarray:
+        dt = np.float64 if precision == 64 else np.float32
+        dx = DeviceArray.from_host(np.asarray(x, dtype=dt))
+        dy = DeviceArray(dx.n, dt)
+        self.mult(dx, dy)
+        out = dy.download()
+        dx.free()
+        dy.free()
+        return out
+
+    def spmv_host(self, u: np.ndarray, Yu: np.ndarray) -> None:
+        """MFStokesWrapper_A11_HIP contract on host arrays: Yu += A u."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert Yu.dtype == np.float64 and Yu.flags.c_contiguous
+        call("ptmf_a11_spmv_host", self.handle, _lib._hp(u), _lib._hp(Yu))
+
+    def diagonal_host(self) -> np.ndarray:
+        d = DeviceArray(3 * self.nnodes)
+        self.get_diagonal(d)
+        out = d.download()
+        d.free()
+        return out
+
+    def time_mult(self, x: DeviceArray, y: DeviceArray, reps: int) -> float:
+        """ms per MatMult over `reps` back-to-back applies (HIP events)."""
+        prec = 32 if x.dtype == np.float32 else 64
+        a = C.c_double(0)
+        call("ptmf_a11_time", self.handle, prec, x.ptr, y.ptr, int(reps), C.byref(a))
+        return a.value
+
+    def time_kernel(self, x: DeviceArray, y: DeviceArray, reps: int) -> float:
+        """ms per launch of the element kernel alone (HIP events)."""
+        prec = 32 if x.dtype == np.float32 else 64
+        a = C.c_double(0)
+        call("ptmf_a11_time_kernel", self.handle, prec, x.ptr, y.ptr, int(reps), C.byref(a))
+        return a.value
+
+    def destroy(self) -> None:
+        if self.handle is not None and self.handle.value:
+            call("ptmf_op_destroy", C.byref(self.handle))
+        self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle is not None and self.handle.value and _lib._lib is not None:
+                _lib._lib.ptmf_op_destroy(C.byref(self.handle))
+        except Exception:
+            pass
+
+
+class MatStokesMF(MatA11MF):
+    """Adds the P1disc pressure couplings of the Stokes operator."""
+
+    def mult_a12(self, p: DeviceArray, yu: DeviceArray) -> None:
+        """MatMult_MFStokes_A12: yu = B p, Dirichlet velocity rows zeroed."""
+        call("ptmf_a12_mult", self.handle, p.ptr, yu.ptr)
+
+    def mult_a21(self, u: DeviceArray, yp: DeviceArray) -> None:
+        """MatMult_MFStokes_A21: yp = B^T mask(u)."""
+        call("ptmf_a21_mult", self.handle, u.ptr, yp.ptr)
+
+    def a12_host(self, p: np.ndarray) -> np.ndarray:
+        dp = DeviceArray.from_host(np.asarray(p, dtype=np.float64))
+        dy = DeviceArray(3 * self.nnodes)
+        self.mult_a12(dp, dy)
+        out = dy.download()
+        dp.free()
+        dy.free()
+        return out
+
+    def a21_host(self, u: np.ndarray) -> np.ndarray:
+        du = DeviceArray.from_host(np.asarray(u, dtype=np.float64))
+        dy = DeviceArray(4 * self.nel)
+        self.mult_a21(du, dy)
+        out = dy.download()
+        du.free()
+        dy.free()
+        return out

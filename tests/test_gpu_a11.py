@@ -1,0 +1,150 @@
+"""GPU parity of the A11 operator stack against the CPU oracle, through the C-ABI.
+
+Tolerances (FP64): ||y_gpu - y_oracle||_inf / ||y_oracle||_inf <= 1e-12.
+FP32 apply: <= 5e-5 relative to the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ptatin3d_amd import fields as F
+from ptatin3d_amd._lib import DeviceArray, PtmfError
+from ptatin3d_amd.operators import MatA11MF, MatStokesMF
+
+from conftest import rel_inf
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+TOL32 = 5e-5
+
+
+def _case(m, perturb=0.1, lo=1e-2, hi=1e2, seed=5):
+    rng = np.random.default_rng(seed)
+    coords = F.box_coords(*m)
+    if perturb:
+        coords = F.perturb_interior(coords, *m, perturb, rng)
+    E = m[0] * m[1] * m[2]
+    eta = F.eta_loguniform(E, lo, hi, rng)
+    bc = F.dirichlet_all_faces(*m)
+    x = rng.uniform(-1, 1, 3 * F.nnodes(*m))
+    return coords, eta, bc, x
+
+
+def _op(m, coords, eta, bc, cls=MatA11MF):
+    return cls(*m).setup(coords, eta, bc)
+
+
+def test_config1_matmult_parity(gpu):
+    c = F.config1()
+    m = (c["mx"], c["my"], c["mz"])
+    A = _op(m, c["coords"], c["eta"], c["bc"])
+    y = A.mult_host(c["u"])
+    yo = O.a11_matmult(*m, c["coords"], c["eta"], c["bc"], c["u"])
+    assert rel_inf(y, yo) <= TOL64
+    A.destroy()
+
+
+@pytest.mark.parametrize("m", [(1, 1, 1), (2, 1, 1), (3, 5, 7), (7, 3, 2), (5, 4, 6), (22, 1, 3)])
+def test_ragged_boxes(gpu, m):
+    coords, eta, bc, x = _case(m)
+    A = _op(m, coords, eta, bc)
+    y = A.mult_host(x)
+    yo = O.a11_matmult(*m, coords, eta, bc, x)
+    assert rel_inf(y, yo) <= TOL64
+    # no-BC spmv contract: Yu += A u on host arrays
+    Yu0 = np.random.default_rng(1).uniform(-1, 1, x.size)
+    Yu = Yu0.copy()
+    A.spmv_host(x, Yu)
+    assert rel_inf(Yu - Yu0, O.a11_tensor(*m, coords, eta, x)) <= TOL64
+    A.destroy()
+
+
+def test_reference_test_vector(gpu):
+    m = (8, 9, 11)  # the reference's operator-test mesh
+    coords, eta, bc, _ = _case(m, perturb=0.0, lo=1.0, hi=10.0)
+    f = F.reference_test_vector(coords)
+    A = _op(m, coords, eta, bc)
+    assert rel_inf(A.mult_host(f), O.a11_matmult(*m, coords, eta, bc, f)) <= TOL64
+    A.destroy()
+
+
+def test_config2_parity_and_properties(gpu):
+    c = F.config2()
+    m = (c["mx"], c["my"], c["mz"])
+    A = _op(m, c["coords"], c["eta"], c["bc"])
+    x = c["u"]
+    dx = DeviceArray.from_host(x)
+    dy = DeviceArray(x.size)
+    A.mult(dx, dy)
+    y = dy.download()
+    yo = O.a11_matmult(*m, c["coords"], c["eta"], c["bc"], x)
+    assert rel_inf(y, yo) <= TOL64
+    # repeat apply: atomics reorder sums only at rounding level
+    A.mult(dx, dy)
+    assert rel_inf(dy.download(), y) <= 1e-14
+    # symmetry on the free space
+    z = np.random.default_rng(9).uniform(-1, 1, x.size)
+    z[c["bc"].astype(bool)] = 0.0
+    Az = A.mult_host(z)
+    xm = x.copy()
+    xm[c["bc"].astype(bool)] = 0.0
+    assert abs(z @ y - xm @ Az) <= 1e-12 * abs(z @ y)
+    A.destroy()
+
+
+def test_fp32_apply(gpu):
+    c = F.config1(m=12)
+    m = (c["mx"], c["my"], c["mz"])
+    A = _op(m, c["coords"], c["eta"], c["bc"])
+    y32 = A.mult_host(c["u"], precision=32)
+    yo = O.a11_matmult(*m, c["coords"], c["eta"], c["bc"], c["u"])
+    assert y32.dtype == np.float32
+    assert rel_inf(y32, yo) <= TOL32
+    A.destroy()
+
+
+def test_diagonal_parity(gpu):
+    for m, pert in (((6, 5, 4), 0.1), ((16, 16, 16), 0.1)):
+        coords, eta, bc, _ = _case(m, perturb=pert)
+        A = _op(m, coords, eta, bc)
+        d = A.diagonal_host()
+        do = O.a11_diagonal(*m, coords, eta, bc)
+        assert rel_inf(d, do) <= TOL64
+        A.destroy()
+
+
+def test_a12_a21_parity(gpu):
+    m = (5, 6, 4)
+    coords, eta, bc, u = _case(m)
+    E = m[0] * m[1] * m[2]
+    p = np.random.default_rng(2).uniform(-1, 1, 4 * E)
+    S = _op(m, coords, eta, bc, MatStokesMF)
+    assert rel_inf(S.a12_host(p), O.a12_matmult(*m, coords, bc, p)) <= TOL64
+    assert rel_inf(S.a21_host(u), O.a21_matmult(*m, coords, bc, u)) <= TOL64
+    S.destroy()
+
+
+def test_not_setup_is_an_error(gpu):
+    A = MatA11MF(2, 2, 2)
+    x = DeviceArray(3 * A.nnodes)
+    y = DeviceArray(3 * A.nnodes)
+    with pytest.raises(PtmfError) as ei:
+        A.mult(x, y)
+    assert ei.value.code == 2 and "not setup" in str(ei.value)
+    with pytest.raises(PtmfError):
+        MatA11MF(2, 2, 2, a11_op="cuda")
+    A.destroy()
+
+
+def test_coefficient_state_bumps(gpu):
+    m = (2, 2, 2)
+    coords, eta, bc, x = _case(m)
+    A = _op(m, coords, eta, bc)
+    s0 = A.state
+    y0 = A.mult_host(x)
+    A.set_viscosity(2.0 * eta)
+    assert A.state == s0 + 1
+    y1 = A.mult_host(x)
+    free = bc == 0
+    assert rel_inf(y1[free], 2.0 * y0[free]) <= TOL64
+    A.destroy()
