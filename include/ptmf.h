@@ -78,6 +78,12 @@ int ptmf_op_set_dirichlet(ptmf_op op, const unsigned char *mask);
 /* device-array variants (no host round trip) */
 int ptmf_op_set_coordinates_device(ptmf_op op, const double *coords);
 int ptmf_op_set_viscosity_device(ptmf_op op, const double *eta);
+/* Work decomposition of the MatMult kernel: TX x TY element tiles (TX*TY <= 21)
+ * marching through ZC element layers.  A default is chosen at creation; the
+ * autotuner overrides it.  No reference counterpart (the CUDA backend's
+ * coloring, stokes_operators_cuda.cu:337-397, plays this role there). */
+int ptmf_op_set_tiling(ptmf_op op, int TX, int TY, int ZC);
+int ptmf_op_get_tiling(ptmf_op op, int *TX, int *TY, int *ZC);
 /* coefficient state counter, bumped by every set_* call (replaces the
  * PetscObjectState stamp of stokes_operators.c:1162-1163) */
 int ptmf_op_state(ptmf_op op, unsigned long *state);

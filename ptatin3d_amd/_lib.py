@@ -49,6 +49,8 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_op_set_coordinates_device": (_I, [_P, _P]),
     "ptmf_op_set_viscosity_device": (_I, [_P, _P]),
     "ptmf_op_state": (_I, [_P, C.POINTER(C.c_ulong)]),
+    "ptmf_op_set_tiling": (_I, [_P, _I, _I, _I]),
+    "ptmf_op_get_tiling": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "ptmf_a11_spmv": (_I, [_P, _P, _P]),
     "ptmf_a11_spmv_host": (_I, [_P, _P, _P]),
     "ptmf_a11_mult": (_I, [_P, _P, _P]),

@@ -1,26 +1,39 @@
 // a11.hip -- matrix-free Q2 viscous block (A11) apply for gfx950.
 //
-// Computes y += A11_local u for all elements of a structured Q2 box, the
-// operator of MFStokesWrapper_A11_Tensor (stokes_operators_tensor.c:181-288):
-// per element and quadrature point, grad u and the Jacobian from the 27
-// nodal values, then w*detJ*(2 eta sym(grad u)) G^T, then the transposed
-// gradient back to the nodes.
+// Computes A11_local u for all elements of a structured Q2 box, the operator
+// of MFStokesWrapper_A11_Tensor (stokes_operators_tensor.c:181-288): per
+// element and quadrature point, grad u and the Jacobian from the 27 nodal
+// values, then w*detJ*(2 eta sym(grad u)) G^T, then the transposed gradient
+// back to the nodes.
 //
-// Work decomposition (MI355X-first, not the reference's one-warp-per-element):
+// Element work decomposition (MI355X-first, not one-warp-per-element):
 //   * 3 lanes per element, lane c owns quadrature plane zeta_c (9 qps) and,
 //     on input/output, node plane k = c of the element; 21 elements per
 //     64-lane wave, one wave per work-group (no cross-wave barriers).
 //   * Collocated sum factorisation: nodal -> quadrature-grid interpolation
 //     (B x B x B), then the three reference derivatives on the quadrature
 //     grid with the 3x3 collocation matrix C = D B^-1.  The xi/eta stages
-//     are in registers; only the zeta stage crosses lanes, through LDS
-//     (9 values per field per lane instead of 27).
+//     are in registers; only the zeta stage crosses lanes, through LDS.
 //   * The 1/detJ of the inverse Jacobian is folded into one scalar per qp
-//     (adjugate form), so per qp there is one division.
-//   * Scatter: hardware FP64/FP32 atomic add into y (global_atomic_add_f64);
-//     y is cleared beforehand by the caller.
+//     (adjugate form): one division per qp.
 //   * Viscosity is stored quadrature-point-major (eta[q*nel + e]) so the
-//     lanes of a wave read consecutive elements.
+//     lanes of a wave read consecutive elements; each lane loads its 9
+//     values before the arithmetic starts.
+//
+// Scatter (MatMult path, a11_tile_kernel + a11_face_kernel):
+//   * A wave owns a TX x TY tile of elements in x-y (TX*TY <= 21) and marches
+//     through a chunk of ZC element layers in z.  Element outputs are summed
+//     into an LDS window of three node planes (LDS float atomics); after each
+//     layer the two completed node planes are flushed with plain stores.
+//   * A tile owns its node range [2x0,2x1) x [2y0,2y1) x [2z0,2z1) (plus the
+//     closing planes at the domain's upper boundary).  Contributions to the
+//     tile's upper faces go to per-tile face buffers (x face first, then y,
+//     then z), and a11_face_kernel adds them into the owning tile's lower
+//     faces.  No global atomics, no clearing of y, deterministic sums.
+//   * Dirichlet rows (y = x) are written last, over the compact list of
+//     Dirichlet dofs.
+// The spmv path (y += A u, no boundary handling) keeps a one-pass kernel with
+// global atomic adds; it serves the host-array PETSc adapter contract.
 #include "ptmf_internal.h"
 
 namespace ptmf {
@@ -28,6 +41,7 @@ namespace {
 
 constexpr int kEPW = 21;  // elements per wave (3 lanes each; lane 63 idle)
 constexpr int kSlots = 54;
+constexpr int kWinNodes = 130;  // >= (2TX+1)(2TY+1) for every TX*TY <= 21
 
 template <typename T>
 __device__ __forceinline__ T sel3(int c, double a0, double a1, double a2) {
@@ -70,24 +84,25 @@ __device__ __forceinline__ void load_interp_plane(const T *__restrict__ f, long 
   }
 }
 
+// The element computation of one lane: element e (node plane origin `base`,
+// lane plane cp), result out[l][3j+i] = (A11_e u)(node (i,j,k=cp), comp l).
+// Every lane of the wave must call it (it contains wave-wide barriers); lanes
+// with writer == false (lane 63) never touch the exchange buffer.
 template <typename T, bool kMask>
-__global__ void __launch_bounds__(64)
-    a11_apply_kernel(int mx, int my, long nel, long X, long XY, const T *__restrict__ xc,
-                     const T *__restrict__ eta, const uint8_t *__restrict__ bcn,
-                     const T *__restrict__ u, T *__restrict__ y) {
-  __shared__ T sh[3][kSlots][kEPW];
-
-  const int lane = threadIdx.x;
-  const int c = lane / kEPW;  // quadrature / node plane; 3 marks the idle lane
-  const int le = lane - c * kEPW;
-  const int cp = c < 3 ? c : 2;
-  long e = (long)blockIdx.x * kEPW + le;
-  const bool live = (c < 3) && (e < nel);
-  if (e >= nel) e = nel - 1;
-  const bool writer = c < 3;
-
-  const long ei = e % mx, er = e / mx, ej = er % my, ek = er / my;
-  const long base = 2 * ei + X * (2 * ej) + XY * (2 * ek + cp);
+__device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer, long e, long nel,
+                                                    long base, long X, const T *__restrict__ xc,
+                                                    const T *__restrict__ eta,
+                                                    const uint8_t *__restrict__ bcn,
+                                                    const T *__restrict__ u,
+                                                    T (*sh)[kSlots][kEPW], T out[3][9]) {
+  // viscosity of this lane's 9 quadrature points, issued before any arithmetic
+  T ev[9];
+  {
+    const T *etap = eta + (long)(9 * cp) * nel + e;
+#pragma unroll
+    for (int p = 0; p < 9; p++) ev[p] = etap[(long)p * nel];
+  }
+  __builtin_amdgcn_sched_barrier(0);
 
   // zeta tables of this lane's plane: forward B[c][k], D[c][k]; backward
   // (transposed) B[c'][k=c], D[c'][k=c].
@@ -151,7 +166,6 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
     for (int p = 0; p < 9; p++) W[l][p] = T(0);
 
-  const T *etap = eta + (long)(9 * cp) * nel + e;
 #pragma unroll
   for (int b = 0; b < 3; b++)
 #pragma unroll
@@ -183,7 +197,7 @@ __global__ void __launch_bounds__(64)
       A[2][2] = M[0][0] * M[1][1] - M[1][0] * M[0][1];
       const T det = M[0][0] * A[0][0] + M[0][1] * A[1][0] + M[0][2] * A[2][0];
       const T wq = (T)(kW[a] * kW[b]) * wz;
-      const T s = (T(2) * wq * etap[(long)p * nel]) / det;
+      const T s = (T(2) * wq * ev[p]) / det;
       // gh[k][l] = det * d u_l / d x_k
       T gh[3][3];
 #pragma unroll
@@ -235,8 +249,10 @@ __global__ void __launch_bounds__(64)
               w2 = sh[2][27 + 9 * l + p][le];
       Y[l][p] = bt[0] * w0 + bt[1] * w1 + bt[2] * w2 + dt[0] * z0 + dt[1] * z1 + dt[2] * z2;
     }
+  // the exchange buffer is reused by the caller's next element
+  __syncthreads();
 
-  // ---- backward in-plane: transposed interpolation, then scatter-add
+  // ---- backward in-plane: transposed interpolation
 #pragma unroll
   for (int l = 0; l < 3; l++) {
     T t[3][3];  // [b][i]
@@ -249,14 +265,227 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
     for (int j = 0; j < 3; j++)
 #pragma unroll
-      for (int i = 0; i < 3; i++) {
-        const T v = (T)kB[0][j] * t[0][i] + (T)kB[1][j] * t[1][i] + (T)kB[2][j] * t[2][i];
-        if (live) atomicAdd(&y[3 * (base + i + X * j) + l], v);
-      }
+      for (int i = 0; i < 3; i++)
+        out[l][3 * j + i] = (T)kB[0][j] * t[0][i] + (T)kB[1][j] * t[1][i] + (T)kB[2][j] * t[2][i];
   }
 }
 
+// ---------------------------------------------------------------------------
+// spmv: y += A11_local u with global atomic adds (no boundary handling)
+// ---------------------------------------------------------------------------
+template <typename T, bool kMask>
+__global__ void __launch_bounds__(64)
+    a11_atomic_kernel(int mx, int my, long nel, long X, long XY, const T *__restrict__ xc,
+                      const T *__restrict__ eta, const uint8_t *__restrict__ bcn,
+                      const T *__restrict__ u, T *__restrict__ y) {
+  __shared__ T sh[3][kSlots][kEPW];
+  const int lane = threadIdx.x;
+  const int c = lane / kEPW;  // quadrature / node plane; 3 marks the idle lane
+  const int le = lane - c * kEPW;
+  const int cp = c < 3 ? c : 2;
+  long e = (long)blockIdx.x * kEPW + le;
+  const bool live = (c < 3) && (e < nel);
+  if (e >= nel) e = nel - 1;
+  const long ei = e % mx, er = e / mx, ej = er % my, ek = er / my;
+  const long base = 2 * ei + X * (2 * ej) + XY * (2 * ek + cp);
+  T out[3][9];
+  element_plane_apply<T, kMask>(cp, le, c < 3, e, nel, base, X, xc, eta, bcn, u, sh, out);
+  if (live) {
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) atomicAdd(&y[3 * (base + i + X * j) + l], out[l][3 * j + i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MatMult: tiled z-march with an LDS node-plane window and face buffers
+// ---------------------------------------------------------------------------
+struct TileArgs {
+  int mx, my, mz;
+  long X, XY, nel;
+  int TX, TY, ZC, ntx, nty;
+  long fbx_n, fby_n, fbz_n;  // per-tile face buffer sizes in nodes
+};
+
+struct TileBox {
+  int tx, ty, tz, x0, y0, z0, nxt, nyt, nzt;
+  bool xend, yend, zend;
+};
+
+__device__ __forceinline__ TileBox tile_box(const TileArgs &a, long tile) {
+  TileBox t;
+  t.tx = (int)(tile % a.ntx);
+  t.ty = (int)((tile / a.ntx) % a.nty);
+  t.tz = (int)(tile / ((long)a.ntx * a.nty));
+  t.x0 = t.tx * a.TX;
+  t.y0 = t.ty * a.TY;
+  t.z0 = t.tz * a.ZC;
+  t.nxt = min(a.TX, a.mx - t.x0);
+  t.nyt = min(a.TY, a.my - t.y0);
+  t.nzt = min(a.ZC, a.mz - t.z0);
+  t.xend = t.x0 + t.nxt == a.mx;
+  t.yend = t.y0 + t.nyt == a.my;
+  t.zend = t.z0 + t.nzt == a.mz;
+  return t;
+}
+
+// Write the completed node plane P of the window: owned nodes to y, the
+// tile's upper-face nodes to the face buffers (x face first, then y, then z).
+// Pure LDS reads and global stores; clears the window entries it reads.
+template <typename T>
+__device__ __forceinline__ void flush_plane(const TileArgs &a, long tile, const TileBox &tb, int P,
+                                            T *__restrict__ win, T *__restrict__ y,
+                                            T *__restrict__ fbx, T *__restrict__ fby,
+                                            T *__restrict__ fbz) {
+  const int wX = 2 * tb.nxt + 1, wY = 2 * tb.nyt + 1;
+  const int W3 = 3 * wX;
+  const float inv = 1.0f / (float)W3;
+  const int kl = P - 2 * tb.z0;
+  const bool oz = (kl < 2 * tb.nzt) || tb.zend;
+  const long rowbase = (long)(2 * tb.x0) + a.XY * (long)P;
+  for (int t = threadIdx.x; t < W3 * wY; t += 64) {
+    int jl = (int)((float)t * inv);
+    if (jl * W3 > t) jl--;
+    if ((jl + 1) * W3 <= t) jl++;
+    const int r = t - jl * W3;
+    const int il = r / 3, l = r - 3 * il;
+    const T v = win[t];
+    win[t] = T(0);
+    const bool ox = (il < 2 * tb.nxt) || tb.xend;
+    const bool oy = (jl < 2 * tb.nyt) || tb.yend;
+    if (!ox) {
+      fbx[3 * (tile * a.fbx_n + jl + (long)(2 * a.TY + 1) * kl) + l] = v;
+    } else if (!oy) {
+      fby[3 * (tile * a.fby_n + il + (long)(2 * a.TX + 1) * kl) + l] = v;
+    } else if (!oz) {
+      fbz[3 * (tile * a.fbz_n + il + (long)(2 * a.TX + 1) * jl) + l] = v;
+    } else {
+      y[3 * (rowbase + il + a.X * (long)(2 * tb.y0 + jl)) + l] = v;
+    }
+  }
+}
+
+template <typename T, bool kMask>
+__global__ void __launch_bounds__(64)
+    a11_tile_kernel(TileArgs a, const T *__restrict__ xc, const T *__restrict__ eta,
+                    const uint8_t *__restrict__ bcn, const T *__restrict__ u, T *__restrict__ y,
+                    T *__restrict__ fbx, T *__restrict__ fby, T *__restrict__ fbz) {
+  __shared__ T sh[3][kSlots][kEPW];
+  __shared__ T win[3][3 * kWinNodes];
+  const long tile = blockIdx.x;
+  const TileBox tb = tile_box(a, tile);
+  const int lane = threadIdx.x;
+  const int c = lane / kEPW;
+  const int le = lane - c * kEPW;
+  const int cp = c < 3 ? c : 2;
+  const int lx = le % a.TX, ly = le / a.TX;
+  const bool live = (c < 3) && (le < a.TX * a.TY) && (lx < tb.nxt) && (ly < tb.nyt);
+  const int ex = tb.x0 + min(lx, tb.nxt - 1), ey = tb.y0 + min(ly, tb.nyt - 1);
+  const int wX = 2 * tb.nxt + 1;
+
+  for (int t = lane; t < 9 * kWinNodes; t += 64) (&win[0][0])[t] = T(0);
+  __syncthreads();
+
+  for (int kz = 0; kz < tb.nzt; kz++) {
+    const long ek = tb.z0 + kz;
+    const long e = ex + (long)a.mx * (ey + (long)a.my * ek);
+    const long base = 2L * ex + a.X * (2L * ey) + a.XY * (2 * ek + cp);
+    T out[3][9];
+    element_plane_apply<T, kMask>(cp, le, c < 3, e, a.nel, base, a.X, xc, eta, bcn, u, sh, out);
+    if (live) {
+      T *w = win[(2 * kz + cp) % 3];
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+          for (int l = 0; l < 3; l++)
+            atomicAdd(&w[3 * ((2 * ly + j) * wX + 2 * lx + i) + l], out[l][3 * j + i]);
+    }
+    __syncthreads();
+    flush_plane<T>(a, tile, tb, (int)(2 * ek), win[(2 * kz) % 3], y, fbx, fby, fbz);
+    flush_plane<T>(a, tile, tb, (int)(2 * ek + 1), win[(2 * kz + 1) % 3], y, fbx, fby, fbz);
+    __syncthreads();
+  }
+  flush_plane<T>(a, tile, tb, 2 * (tb.z0 + tb.nzt), win[(2 * tb.nzt) % 3], y, fbx, fby, fbz);
+}
+
+// Adds the neighbours' face-buffer contributions into the nodes on the lower
+// faces of each tile.
+template <typename T>
+__global__ void __launch_bounds__(64)
+    a11_face_kernel(TileArgs a, const T *__restrict__ fbx,
+                    const T *__restrict__ fby, const T *__restrict__ fbz, T *__restrict__ y) {
+  const long tile = blockIdx.x;
+  const TileBox tb = tile_box(a, tile);
+  const int oX = 2 * tb.nxt + (tb.xend ? 1 : 0);
+  const int oY = 2 * tb.nyt + (tb.yend ? 1 : 0);
+  const int oZ = 2 * tb.nzt + (tb.zend ? 1 : 0);
+  const bool hx = tb.tx > 0, hy = tb.ty > 0, hz = tb.tz > 0;
+  const int nX = hx ? oY * oZ : 0;
+  const int iy0 = hx ? 1 : 0, jz0 = hy ? 1 : 0, iz0 = hx ? 1 : 0;
+  const int nY = hy ? (oX - iy0) * oZ : 0;
+  const int nZ = hz ? (oX - iz0) * (oY - jz0) : 0;
+  for (int t = threadIdx.x; t < nX + nY + nZ; t += 64) {
+    int il, jl, kl;
+    if (t < nX) {
+      il = 0; jl = t % oY; kl = t / oY;
+    } else if (t < nX + nY) {
+      const int r = t - nX, w = oX - iy0;
+      il = iy0 + r % w; jl = 0; kl = r / w;
+    } else {
+      const int r = t - nX - nY, w = oX - iz0;
+      il = iz0 + r % w; jl = jz0 + r / w; kl = 0;
+    }
+    const int I = 2 * tb.x0 + il, J = 2 * tb.y0 + jl, K = 2 * tb.z0 + kl;
+    T acc[3] = {T(0), T(0), T(0)};
+    for (int dz = (kl == 0 && hz) ? -1 : 0; dz <= 0; dz++)
+      for (int dy = (jl == 0 && hy) ? -1 : 0; dy <= 0; dy++)
+        for (int dx = (il == 0 && hx) ? -1 : 0; dx <= 0; dx++) {
+          if (dx == 0 && dy == 0 && dz == 0) continue;
+          const long tp = (tb.tx + dx) + (long)a.ntx * ((tb.ty + dy) + (long)a.nty * (tb.tz + dz));
+          const int ilp = I - 2 * (tb.tx + dx) * a.TX, jlp = J - 2 * (tb.ty + dy) * a.TY,
+                    klp = K - 2 * (tb.tz + dz) * a.ZC;
+          const T *src;
+          if (dx == -1)
+            src = fbx + 3 * (tp * a.fbx_n + jlp + (long)(2 * a.TY + 1) * klp);
+          else if (dy == -1)
+            src = fby + 3 * (tp * a.fby_n + ilp + (long)(2 * a.TX + 1) * klp);
+          else
+            src = fbz + 3 * (tp * a.fbz_n + ilp + (long)(2 * a.TX + 1) * jlp);
+          acc[0] += src[0];
+          acc[1] += src[1];
+          acc[2] += src[2];
+        }
+    const long n = (long)I + a.X * (long)J + a.XY * (long)K;
+#pragma unroll
+    for (int l = 0; l < 3; l++) y[3 * n + l] += acc[l];
+  }
+}
+
+TileArgs tile_args(const ptmf_op_s *op) {
+  const Box &b = op->box;
+  TileArgs a;
+  a.mx = b.mx; a.my = b.my; a.mz = b.mz;
+  a.X = b.X; a.XY = b.X * b.Y; a.nel = b.nel;
+  a.TX = op->tiling.TX; a.TY = op->tiling.TY; a.ZC = op->tiling.ZC;
+  a.ntx = op->tiling.ntx; a.nty = op->tiling.nty;
+  a.fbx_n = (long)(2 * a.TY + 1) * (2 * a.ZC + 1);
+  a.fby_n = (long)(2 * a.TX + 1) * (2 * a.ZC + 1);
+  a.fbz_n = (long)(2 * a.TX + 1) * (2 * a.TY + 1);
+  return a;
+}
+
 }  // namespace
+
+long face_buffer_doubles(int TX, int TY, int ZC, long ntiles) {
+  const long per = (long)(2 * TY + 1) * (2 * ZC + 1) + (long)(2 * TX + 1) * (2 * ZC + 1) +
+                   (long)(2 * TX + 1) * (2 * TY + 1);
+  return 3 * per * ntiles;
+}
 
 template <typename T>
 int launch_a11(const ptmf_op_s *op, const T *u, T *y, bool mask_columns) {
@@ -266,16 +495,41 @@ int launch_a11(const ptmf_op_s *op, const T *u, T *y, bool mask_columns) {
   const long nblk = (b.nel + kEPW - 1) / kEPW;
   PTMF_CHECK(nblk < (1L << 31), PTMF_ERR_ARG, "a11: mesh too large (%ld blocks)", nblk);
   if (mask_columns)
-    a11_apply_kernel<T, true><<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
+    a11_atomic_kernel<T, true><<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
         b.mx, b.my, b.nel, b.X, b.X * b.Y, d.coords, d.eta, op->bcnode, u, y);
   else
-    a11_apply_kernel<T, false><<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
+    a11_atomic_kernel<T, false><<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
         b.mx, b.my, b.nel, b.X, b.X * b.Y, d.coords, d.eta, nullptr, u, y);
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }
 
+template <typename T>
+int launch_a11_mult(const ptmf_op_s *op, const T *x, T *y, bool faces) {
+  const OpData<T> &d = *(const OpData<T> *)(sizeof(T) == 8 ? (const void *)&op->d64
+                                                           : (const void *)&op->d32);
+  const Tiling &tl = op->tiling;
+  PTMF_CHECK(tl.TX * tl.TY <= kEPW && (2 * tl.TX + 1) * (2 * tl.TY + 1) <= kWinNodes && tl.ZC > 0,
+             PTMF_ERR_ARG, "a11: invalid tiling %d x %d x %d", tl.TX, tl.TY, tl.ZC);
+  PTMF_CHECK(op->faces != nullptr, PTMF_ERR_NOT_SETUP, "a11: face buffers not allocated");
+  const TileArgs a = tile_args(op);
+  T *fb = (T *)op->faces;
+  const long ntiles = tl.ntiles;
+  T *fbx = fb, *fby = fbx + 3 * a.fbx_n * ntiles, *fbz = fby + 3 * a.fby_n * ntiles;
+  a11_tile_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
+      a, d.coords, d.eta, op->bcnode, x, y, fbx, fby, fbz);
+  PTMF_HIP(hipGetLastError());
+  if (faces) {
+    a11_face_kernel<T><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(a, fbx, fby, fbz, y);
+    PTMF_HIP(hipGetLastError());
+    PTMF_TRY(launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
+  }
+  return PTMF_OK;
+}
+
 template int launch_a11<double>(const ptmf_op_s *, const double *, double *, bool);
 template int launch_a11<float>(const ptmf_op_s *, const float *, float *, bool);
+template int launch_a11_mult<double>(const ptmf_op_s *, const double *, double *, bool);
+template int launch_a11_mult<float>(const ptmf_op_s *, const float *, float *, bool);
 
 }  // namespace ptmf

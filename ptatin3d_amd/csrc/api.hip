@@ -22,6 +22,42 @@ int ptmf_set_error(int code, const char *fmt, ...) {
 
 using ptmf::Box;
 
+namespace ptmf {
+
+Tiling make_tiling(int mx, int my, int mz, int TX, int TY, int ZC) {
+  Tiling t;
+  t.TX = TX; t.TY = TY; t.ZC = ZC;
+  t.ntx = (mx + TX - 1) / TX;
+  t.nty = (my + TY - 1) / TY;
+  t.ntz = (mz + ZC - 1) / ZC;
+  t.ntiles = (long)t.ntx * t.nty * t.ntz;
+  return t;
+}
+
+Tiling choose_tiling(int mx, int my, int mz) {
+  int bestx = 1, besty = 1;
+  double best = -1.0;
+  for (int TX = 1; TX <= 21; TX++)
+    for (int TY = 1; TX * TY <= 21; TY++) {
+      if ((2 * TX + 1) * (2 * TY + 1) > 130) continue;
+      const long tiles = (long)((mx + TX - 1) / TX) * ((my + TY - 1) / TY);
+      const double fill = (double)mx * my / (tiles * 21.0);
+      const double face = 1.0 / TX + 1.0 / TY;  // face-buffer traffic grows with this
+      const double score = fill - 0.02 * face;
+      if (score > best + 1e-12) {
+        best = score; bestx = TX; besty = TY;
+      }
+    }
+  const long txy = (long)((mx + bestx - 1) / bestx) * ((my + besty - 1) / besty);
+  long zc = (long)mz * txy / 3072;
+  if (zc < 2) zc = 2;
+  if (zc > 16) zc = 16;
+  if (zc > mz) zc = mz;
+  return make_tiling(mx, my, mz, bestx, besty, (int)zc);
+}
+
+}  // namespace ptmf
+
 extern "C" {
 
 const char *ptmf_last_error(void) { return g_err; }
@@ -109,6 +145,12 @@ int ptmf_op_create(int mx, int my, int mz, ptmf_op *out) {
     if (e != hipSuccess) st = ptmf_set_error(PTMF_ERR_DEVICE, "op_create: %s", hipGetErrorString(e));
   }
   if (st == PTMF_OK) st = ptmf_memset(op->bcnode, 0, b.nnodes);
+  if (st == PTMF_OK) {
+    op->tiling = ptmf::choose_tiling(mx, my, mz);
+    op->faces_doubles = ptmf::face_buffer_doubles(op->tiling.TX, op->tiling.TY, op->tiling.ZC,
+                                                  op->tiling.ntiles);
+    st = ptmf_malloc(&op->faces, sizeof(double) * op->faces_doubles);
+  }
   if (st != PTMF_OK) {
     ptmf_op tmp = op;
     ptmf_op_destroy(&tmp);
@@ -128,6 +170,7 @@ int ptmf_op_destroy(ptmf_op *pop) {
   (void)hipFree(op->d32.eta);
   (void)hipFree(op->bcnode);
   (void)hipFree(op->bcidx);
+  (void)hipFree(op->faces);
   (void)hipFree(op->host_stage);
   (void)hipFree(op->host_stage2);
   if (op->ev0) (void)hipEventDestroy(op->ev0);
@@ -142,6 +185,33 @@ int ptmf_op_sizes(ptmf_op op, long *nnodes, long *nel) {
   PTMF_CHECK(op, PTMF_ERR_ARG, "op_sizes: null op");
   if (nnodes) *nnodes = op->box.nnodes;
   if (nel) *nel = op->box.nel;
+  return PTMF_OK;
+}
+
+int ptmf_op_set_tiling(ptmf_op op, int TX, int TY, int ZC) {
+  PTMF_CHECK(op, PTMF_ERR_ARG, "set_tiling: null op");
+  PTMF_CHECK(TX > 0 && TY > 0 && ZC > 0 && TX * TY <= 21 && (2 * TX + 1) * (2 * TY + 1) <= 130,
+             PTMF_ERR_ARG, "set_tiling: unsupported tile %d x %d x %d (TX*TY <= 21)", TX, TY, ZC);
+  const Box &b = op->box;
+  ptmf::Tiling t = ptmf::make_tiling(b.mx, b.my, b.mz, TX, TY < b.my ? TY : TY, ZC);
+  const long need = ptmf::face_buffer_doubles(TX, TY, ZC, t.ntiles);
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  if (need > op->faces_doubles) {
+    void *p = nullptr;
+    PTMF_TRY(ptmf_malloc(&p, sizeof(double) * need));
+    PTMF_HIP(hipFree(op->faces));
+    op->faces = p;
+    op->faces_doubles = need;
+  }
+  op->tiling = t;
+  return PTMF_OK;
+}
+
+int ptmf_op_get_tiling(ptmf_op op, int *TX, int *TY, int *ZC) {
+  PTMF_CHECK(op, PTMF_ERR_ARG, "get_tiling: null op");
+  if (TX) *TX = op->tiling.TX;
+  if (TY) *TY = op->tiling.TY;
+  if (ZC) *ZC = op->tiling.ZC;
   return PTMF_OK;
 }
 
@@ -244,11 +314,7 @@ int a11_mult(ptmf_op_s *op, const T *x, T *y) {
   PTMF_TRY(check_ready(op, true, true));
   PTMF_CHECK(x && y && (const void *)x != (const void *)y, PTMF_ERR_ARG,
              "a11_mult: x and y must be distinct device arrays");
-  const long n3 = 3 * op->box.nnodes;
-  PTMF_TRY(ptmf::launch_fill<T>(y, T(0), n3, op->stream));
-  PTMF_TRY(ptmf::launch_a11<T>(op, x, y, true));
-  PTMF_TRY(ptmf::launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
-  return PTMF_OK;
+  return ptmf::launch_a11_mult<T>(op, x, y, true);
 }
 
 }  // namespace
@@ -356,10 +422,12 @@ int ptmf_a11_time_kernel(ptmf_op op, int precision, const void *x, void *y, int 
   PTMF_TRY(check_ready(op, true, true));
   if (precision == 64)
     return time_loop(
-        op, reps, [&] { return ptmf::launch_a11<double>(op, (const double *)x, (double *)y, true); },
+        op, reps,
+        [&] { return ptmf::launch_a11_mult<double>(op, (const double *)x, (double *)y, false); },
         ms_per_kernel);
   return time_loop(
-      op, reps, [&] { return ptmf::launch_a11<float>(op, (const float *)x, (float *)y, true); },
+      op, reps,
+      [&] { return ptmf::launch_a11_mult<float>(op, (const float *)x, (float *)y, false); },
       ms_per_kernel);
 }
 

@@ -49,6 +49,20 @@ inline Box make_box(int mx, int my, int mz) {
   return b;
 }
 
+// Work decomposition of the MatMult kernel: TX x TY element tiles in x-y
+// (TX*TY <= 21, one 64-lane wave each) marching through ZC layers in z.
+struct Tiling {
+  int TX = 0, TY = 0, ZC = 0;
+  int ntx = 0, nty = 0, ntz = 0;
+  long ntiles = 0;
+};
+
+// Heuristic tiling: fill the wave's 21 element slots and keep enough waves
+// (about three per SIMD) in flight.
+Tiling choose_tiling(int mx, int my, int mz);
+Tiling make_tiling(int mx, int my, int mz, int TX, int TY, int ZC);
+long face_buffer_doubles(int TX, int TY, int ZC, long ntiles);
+
 // Per-precision device copies of the operator data.
 template <typename T>
 struct OpData {
@@ -70,6 +84,9 @@ struct ptmf_op_s {
   unsigned long state = 0;        // bumped by every coefficient change
   hipStream_t stream = nullptr;   // stream all work of this operator runs on
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ptmf::Tiling tiling;
+  void *faces = nullptr;          // tile face buffers (sized for FP64)
+  long faces_doubles = 0;
   double *host_stage = nullptr;   // device staging for the host-array entry points
   double *host_stage2 = nullptr;
 };
@@ -101,6 +118,10 @@ int ptmf_set_error(int code, const char *fmt, ...);
 namespace ptmf {
 template <typename T>
 int launch_a11(const ptmf_op_s *op, const T *u, T *y, bool mask_columns);
+// MatMult_MFStokes_A11: y = A11 mask(x) on free rows, y = x on Dirichlet rows.
+// faces == false runs the element (tile) kernel alone, for timing.
+template <typename T>
+int launch_a11_mult(const ptmf_op_s *op, const T *x, T *y, bool faces);
 template <typename T>
 int launch_fill(T *x, T v, long n, hipStream_t s);
 template <typename T>

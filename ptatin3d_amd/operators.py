@@ -57,6 +57,15 @@ class MatA11MF:
         assert m.size == 3 * self.nnodes
         call("ptmf_op_set_dirichlet", self.handle, _lib._hp(m))
 
+    def set_tiling(self, TX: int, TY: int, ZC: int) -> None:
+        """Element tile (TX*TY <= 21) and z-chunk of the MatMult kernel."""
+        call("ptmf_op_set_tiling", self.handle, int(TX), int(TY), int(ZC))
+
+    def get_tiling(self) -> tuple[int, int, int]:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        call("ptmf_op_get_tiling", self.handle, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
     @property
     def state(self) -> int:
         s = C.c_ulong(0)
