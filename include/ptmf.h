@@ -51,6 +51,24 @@ extern "C" {
 #define PTMF_ERR_DIVERGED 6   /* Krylov/MG iteration produced NaN/Inf            */
 
 typedef struct ptmf_op_s *ptmf_op;
+typedef struct ptmf_mg_s *ptmf_mg;
+
+/* Options of the geometric-multigrid preconditioner of the viscous block
+ * (the reference configures the same solver through PETSc options:
+ * -fieldsplit_u_mg_levels_ksp_type chebyshev, ..._esteig, ..._max_it,
+ * ..._pc_type jacobi, -fieldsplit_u_mg_coarse_pc_type lu; see
+ * pTatin3dStokesKSPConfigureFSGMG, ptatin_driver_linear_ts.c:819-882). */
+typedef struct {
+  int levels_mg;          /* finest levels in the V-cycle (0: all); coarser levels form
+                             the coarse solver (FP64 PCG + FP64 V-cycle, dense at level 0) */
+  int smoother_precision; /* 32 or 64: operator, Chebyshev and transfers in the V-cycle */
+  int pre_its, post_its;  /* Chebyshev iterations per smoothing (KSP max_it) */
+  double esteig[4];       /* emin = e0*min + e1*max, emax = e2*min + e3*max of D^-1 A
+                             (KSPChebyshevEstEigSet; default 0, 0.2, 0, 1.1) */
+  int esteig_its;         /* Lanczos (Jacobi-PCG) steps of the eigenvalue estimate */
+  double coarse_rtol;     /* relative tolerance of the coarse PCG */
+  int coarse_maxit;
+} ptmf_mg_options;
 
 /* ---- library / device helpers ------------------------------------------ */
 const char *ptmf_last_error(void);
@@ -118,6 +136,44 @@ int ptmf_a11_time(ptmf_op op, int precision, const void *x, void *y, int reps,
  * meaningful afterwards): the duration of the dominant kernel. */
 int ptmf_a11_time_kernel(ptmf_op op, int precision, const void *x, void *y, int reps,
                          double *ms_per_kernel);
+
+/* ---- geometric multigrid on A11 (level 0 = coarsest, as PCMG) ---------- */
+/* Hierarchy of nlevels boxes, each coarsened by 2 in every direction
+ * (pTatin3dStokesBuildMeshHierarchy, ptatin_driver_linear_ts.c:481-513). */
+int ptmf_mg_create(int mx, int my, int mz, int nlevels, ptmf_mg *mg);
+int ptmf_mg_destroy(ptmf_mg *mg);
+int ptmf_mg_default_options(ptmf_mg_options *opt);
+int ptmf_mg_levels(ptmf_mg mg, int *nlevels);
+/* borrowed operator of a level (do not destroy) */
+int ptmf_mg_level_op(ptmf_mg mg, int level, ptmf_op *op);
+/* finest coordinates (host, 3N); coarse levels by injection
+ * (DMDARestrictCoordinatesHierarchy, dmda_project_coords.c:39-71) */
+int ptmf_mg_set_coordinates(ptmf_mg mg, const double *coords);
+/* quadrature-point viscosity of one level (host, 27E, element-major) */
+int ptmf_mg_set_viscosity(ptmf_mg mg, int level, const double *eta);
+/* fill every coarser level from the finest: each coarse element takes the mean
+ * of its 8 children's quadrature values */
+int ptmf_mg_coarsen_viscosity(ptmf_mg mg);
+/* Dirichlet mask of one level (host, 3N bytes) */
+int ptmf_mg_set_dirichlet(ptmf_mg mg, int level, const unsigned char *mask);
+/* diagonals, Jacobi scalings, Chebyshev eigenvalue bounds, coarse factor */
+int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt);
+int ptmf_mg_level_eigs(ptmf_mg mg, int level, double *emin, double *emax);
+int ptmf_mg_set_level_eigs(ptmf_mg mg, int level, double emin, double emax);
+/* one V-cycle on the finest level: x = M^-1 b (device FP64 arrays) */
+int ptmf_mg_vcycle(ptmf_mg mg, const double *b, double *x);
+/* yf += P xc and yc = P^T xf between levels level-1 and level (device FP64) */
+int ptmf_mg_prolong_add(ptmf_mg mg, int level, const double *xc, double *yf);
+int ptmf_mg_restrict(ptmf_mg mg, int level, const double *xf, double *yc);
+/* Chebyshev-Jacobi smoothing on one level (precision 64: double*, 32: float*) */
+int ptmf_mg_smooth(ptmf_mg mg, int level, int precision, int its, int nonzero_guess,
+                   const void *b, void *x);
+/* FGMRES(restart) in FP64 on the finest A11 with the V-cycle as right
+ * preconditioner (-fieldsplit_u_ksp_type fgmres); x holds the initial guess.
+ * history (optional) receives the true residual norm at each restart and the
+ * FGMRES estimate after each iteration. */
+int ptmf_mg_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int restart,
+                   int *its, double *rnorm, double *history, int history_len);
 
 #ifdef __cplusplus
 }

@@ -59,8 +59,40 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_a12_mult": (_I, [_P, _P, _P]),
     "ptmf_a21_mult": (_I, [_P, _P, _P]),
     "ptmf_a11_time": (_I, [_P, _I, _P, _P, _I, _PD]),
+    "ptmf_mg_create": (_I, [_I, _I, _I, _I, C.POINTER(_P)]),
+    "ptmf_mg_destroy": (_I, [C.POINTER(_P)]),
+    "ptmf_mg_default_options": (_I, [_P]),
+    "ptmf_mg_levels": (_I, [_P, C.POINTER(_I)]),
+    "ptmf_mg_level_op": (_I, [_P, _I, C.POINTER(_P)]),
+    "ptmf_mg_set_coordinates": (_I, [_P, _P]),
+    "ptmf_mg_set_viscosity": (_I, [_P, _I, _P]),
+    "ptmf_mg_coarsen_viscosity": (_I, [_P]),
+    "ptmf_mg_set_dirichlet": (_I, [_P, _I, _P]),
+    "ptmf_mg_setup": (_I, [_P, _P]),
+    "ptmf_mg_level_eigs": (_I, [_P, _I, _PD, _PD]),
+    "ptmf_mg_set_level_eigs": (_I, [_P, _I, _D, _D]),
+    "ptmf_mg_vcycle": (_I, [_P, _P, _P]),
+    "ptmf_mg_prolong_add": (_I, [_P, _I, _P, _P]),
+    "ptmf_mg_restrict": (_I, [_P, _I, _P, _P]),
+    "ptmf_mg_smooth": (_I, [_P, _I, _I, _I, _I, _P, _P]),
+    "ptmf_mg_fgmres": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _PD, _PD, _I]),
     "ptmf_a11_time_kernel": (_I, [_P, _I, _P, _P, _I, _PD]),
 }
+
+
+class MGOptions(C.Structure):
+    """ptmf_mg_options (include/ptmf.h)."""
+
+    _fields_ = [
+        ("levels_mg", C.c_int),
+        ("smoother_precision", C.c_int),
+        ("pre_its", C.c_int),
+        ("post_its", C.c_int),
+        ("esteig", C.c_double * 4),
+        ("esteig_its", C.c_int),
+        ("coarse_rtol", C.c_double),
+        ("coarse_maxit", C.c_int),
+    ]
 
 
 class PtmfError(RuntimeError):

@@ -17,14 +17,14 @@
 //   * The 1/detJ of the inverse Jacobian is folded into one scalar per qp
 //     (adjugate form): one division per qp.
 //   * Viscosity is stored quadrature-point-major (eta[q*nel + e]) so the
-//     lanes of a wave read consecutive elements; each lane loads its 9
-//     values before the arithmetic starts.
+//     lanes of a wave read consecutive elements.
 //
 // Scatter (MatMult path, a11_tile_kernel + a11_face_kernel):
 //   * A wave owns a TX x TY tile of elements in x-y (TX*TY <= 21) and marches
 //     through a chunk of ZC element layers in z.  Element outputs are summed
-//     into an LDS window of three node planes (LDS float atomics); after each
-//     layer the two completed node planes are flushed with plain stores.
+//     into an LDS window of three node planes (ordered plain read-add-write,
+//     LDS float atomics are slow here); after each layer the two completed
+//     node planes are flushed with plain stores.
 //   * A tile owns its node range [2x0,2x1) x [2y0,2y1) x [2z0,2z1) (plus the
 //     closing planes at the domain's upper boundary).  Contributions to the
 //     tile's upper faces go to per-tile face buffers (x face first, then y,
@@ -87,7 +87,14 @@ __device__ __forceinline__ void load_interp_plane(const T *__restrict__ f, long 
 // The element computation of one lane: element e (node plane origin `base`,
 // lane plane cp), result out[l][3j+i] = (A11_e u)(node (i,j,k=cp), comp l).
 // Every lane of the wave must call it (it contains wave-wide barriers); lanes
-// with writer == false (lane 63) never touch the exchange buffer.
+// with writer == false (lane 63) never write the exchange buffer.
+//
+// Register budget: the in-plane values Xq, Uq and the accumulator W stay in
+// registers; the zeta derivatives are formed per quadrature point from the
+// exchange buffer, whose slots are then reused for the backward zeta data.
+// A work-group is a single wave, so the LDS operations of all lanes execute in
+// program order: a slot read by every lane at qp p is overwritten only after
+// those reads.
 template <typename T, bool kMask>
 __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer, long e, long nel,
                                                     long base, long X, const T *__restrict__ xc,
@@ -95,29 +102,20 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
                                                     const uint8_t *__restrict__ bcn,
                                                     const T *__restrict__ u,
                                                     T (*sh)[kSlots][kEPW], T out[3][9]) {
-  // viscosity of this lane's 9 quadrature points, issued before any arithmetic
-  T ev[9];
-  {
-    const T *etap = eta + (long)(9 * cp) * nel + e;
-#pragma unroll
-    for (int p = 0; p < 9; p++) ev[p] = etap[(long)p * nel];
-  }
-  __builtin_amdgcn_sched_barrier(0);
-
+  const T *etap = eta + (long)(9 * cp) * nel + e;
   // zeta tables of this lane's plane: forward B[c][k], D[c][k]; backward
   // (transposed) B[c'][k=c], D[c'][k=c].
-  T bz[3], dz[3], bt[3], dt[3];
+  T bz[3], dz[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     bz[k] = sel3<T>(cp, kB[0][k], kB[1][k], kB[2][k]);
     dz[k] = sel3<T>(cp, kD[0][k], kD[1][k], kD[2][k]);
-    bt[k] = sel3<T>(cp, kB[k][0], kB[k][1], kB[k][2]);
-    dt[k] = sel3<T>(cp, kD[k][0], kD[k][1], kD[k][2]);
   }
   const T wz = sel3<T>(cp, kW[0], kW[1], kW[2]);
 
-  // ---- forward: coordinates -> Xq (values on the qp plane), Xz (d/dzeta)
-  T Xq[3][9], Xz[3][9], Uq[3][9], Uz[3][9];
+  // ---- forward: in-plane interpolation of the node planes, exchanged so
+  // every lane sees all three planes (slots 0..26 coordinates, 27..53 velocity)
+  T Xq[3][9], Uq[3][9];
   {
     T V[3][9];
     load_interp_plane<T, false>(xc, base, X, nullptr, V);
@@ -127,17 +125,7 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
 #pragma unroll
         for (int p = 0; p < 9; p++) sh[cp][9 * d + p][le] = V[d][p];
     }
-    __syncthreads();
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-#pragma unroll
-      for (int p = 0; p < 9; p++) {
-        const T v0 = sh[0][9 * d + p][le], v1 = sh[1][9 * d + p][le], v2 = sh[2][9 * d + p][le];
-        Xq[d][p] = bz[0] * v0 + bz[1] * v1 + bz[2] * v2;
-        Xz[d][p] = dz[0] * v0 + dz[1] * v1 + dz[2] * v2;
-      }
   }
-  // ---- forward: velocity (Dirichlet columns masked) -> Uq, Uz
   {
     T V[3][9];
     load_interp_plane<T, kMask>(u, base, X, bcn, V);
@@ -147,17 +135,17 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
 #pragma unroll
         for (int p = 0; p < 9; p++) sh[cp][27 + 9 * d + p][le] = V[d][p];
     }
-    __syncthreads();
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-#pragma unroll
-      for (int p = 0; p < 9; p++) {
-        const T v0 = sh[0][27 + 9 * d + p][le], v1 = sh[1][27 + 9 * d + p][le],
-                v2 = sh[2][27 + 9 * d + p][le];
-        Uq[d][p] = bz[0] * v0 + bz[1] * v1 + bz[2] * v2;
-        Uz[d][p] = dz[0] * v0 + dz[1] * v1 + dz[2] * v2;
-      }
   }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+      Xq[d][p] = bz[0] * sh[0][9 * d + p][le] + bz[1] * sh[1][9 * d + p][le] +
+                 bz[2] * sh[2][9 * d + p][le];
+      Uq[d][p] = bz[0] * sh[0][27 + 9 * d + p][le] + bz[1] * sh[1][27 + 9 * d + p][le] +
+                 bz[2] * sh[2][27 + 9 * d + p][le];
+    }
 
   // ---- pointwise action on the 9 qps of this plane
   T W[3][9];  // in-plane transposed xi/eta derivatives, accumulated
@@ -171,6 +159,7 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       const int p = 3 * b + a;
+      const T etq = etap[(long)p * nel];
       // M[j][d] = d x_d / d xi_j, Gu[j][l] = d u_l / d xi_j
       T M[3][3], Gu[3][3];
 #pragma unroll
@@ -178,11 +167,13 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
         M[0][d] = (T)kC[a][0] * Xq[d][3 * b] + (T)kC[a][1] * Xq[d][3 * b + 1] +
                   (T)kC[a][2] * Xq[d][3 * b + 2];
         M[1][d] = (T)kC[b][0] * Xq[d][a] + (T)kC[b][1] * Xq[d][3 + a] + (T)kC[b][2] * Xq[d][6 + a];
-        M[2][d] = Xz[d][p];
+        M[2][d] = dz[0] * sh[0][9 * d + p][le] + dz[1] * sh[1][9 * d + p][le] +
+                  dz[2] * sh[2][9 * d + p][le];
         Gu[0][d] = (T)kC[a][0] * Uq[d][3 * b] + (T)kC[a][1] * Uq[d][3 * b + 1] +
                    (T)kC[a][2] * Uq[d][3 * b + 2];
         Gu[1][d] = (T)kC[b][0] * Uq[d][a] + (T)kC[b][1] * Uq[d][3 + a] + (T)kC[b][2] * Uq[d][6 + a];
-        Gu[2][d] = Uz[d][p];
+        Gu[2][d] = dz[0] * sh[0][27 + 9 * d + p][le] + dz[1] * sh[1][27 + 9 * d + p][le] +
+                   dz[2] * sh[2][27 + 9 * d + p][le];
       }
       // adjugate A = det * M^-1 (A[k][j] = det * d xi_j / d x_k)
       T A[3][3];
@@ -197,7 +188,7 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
       A[2][2] = M[0][0] * M[1][1] - M[1][0] * M[0][1];
       const T det = M[0][0] * A[0][0] + M[0][1] * A[1][0] + M[0][2] * A[2][0];
       const T wq = (T)(kW[a] * kW[b]) * wz;
-      const T s = (T(2) * wq * ev[p]) / det;
+      const T s = (T(2) * wq * etq) / det;
       // gh[k][l] = det * d u_l / d x_k
       T gh[3][3];
 #pragma unroll
@@ -215,6 +206,8 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
       sg[0][1] = sg[1][0] = hs * (gh[0][1] + gh[1][0]);
       sg[0][2] = sg[2][0] = hs * (gh[0][2] + gh[2][0]);
       sg[1][2] = sg[2][1] = hs * (gh[1][2] + gh[2][1]);
+      // every lane has read slot p of all planes; reuse it for the zeta data
+      __builtin_amdgcn_wave_barrier();
       // dv[j][l] = w detJ sum_m sigma[m][l] d xi_j / d x_m
 #pragma unroll
       for (int l = 0; l < 3; l++) {
@@ -228,6 +221,7 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
         }
         if (writer) sh[cp][9 * l + p][le] = dv2;  // zeta component, exchanged below
       }
+      __builtin_amdgcn_wave_barrier();
     }
   __syncthreads();
   if (writer) {
@@ -239,6 +233,12 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
   __syncthreads();
 
   // ---- backward zeta stage: node plane k = c gathers all three qp planes
+  T bt[3], dt[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    bt[k] = sel3<T>(cp, kB[k][0], kB[k][1], kB[k][2]);
+    dt[k] = sel3<T>(cp, kD[k][0], kD[k][1], kD[k][2]);
+  }
   T Y[3][9];
 #pragma unroll
   for (int l = 0; l < 3; l++)
@@ -395,15 +395,26 @@ __global__ void __launch_bounds__(64)
     const long base = 2L * ex + a.X * (2L * ey) + a.XY * (2 * ek + cp);
     T out[3][9];
     element_plane_apply<T, kMask>(cp, le, c < 3, e, a.nel, base, a.X, xc, eta, bcn, u, sh, out);
-    if (live) {
-      T *w = win[(2 * kz + cp) % 3];
+    // Sum into the window.  For a fixed node slot (i,j) no two lanes touch
+    // the same window entry, so the nine slots are added one after another
+    // with plain read-add-write; the wave executes LDS operations in order
+    // and the wave barriers keep the compiler from interleaving the steps.
+    {
+      T *w = win[(2 * kz + cp) % 3] + 3 * ((2 * ly) * wX + 2 * lx);
 #pragma unroll
       for (int j = 0; j < 3; j++)
 #pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-          for (int l = 0; l < 3; l++)
-            atomicAdd(&w[3 * ((2 * ly + j) * wX + 2 * lx + i) + l], out[l][3 * j + i]);
+        for (int i = 0; i < 3; i++) {
+          if (live) {
+            T *q = w + 3 * (j * wX + i);
+            const T q0 = q[0], q1 = q[1], q2 = q[2];
+            q[0] = q0 + out[0][3 * j + i];
+            q[1] = q1 + out[1][3 * j + i];
+            q[2] = q2 + out[2][3 * j + i];
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
     }
     __syncthreads();
     flush_plane<T>(a, tile, tb, (int)(2 * ek), win[(2 * kz) % 3], y, fbx, fby, fbz);

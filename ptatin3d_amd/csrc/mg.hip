@@ -1,0 +1,1030 @@
+// mg.hip -- geometric multigrid on the matrix-free A11 operator: transfers,
+// Chebyshev-Jacobi smoothing, coarse solve, V-cycle and FP64 FGMRES.
+//
+// Mirrors the reference's viscous-block preconditioner
+// (pTatin3dStokesKSPConfigureFSGMG, ptatin_driver_linear_ts.c:819-882):
+// a PCMG V-cycle whose levels are matrix-free A11 operators, KSPCHEBYSHEV +
+// PCJACOBI smoothers (recurrence of ksp_chebyrn.c:322-411), DMDA Q1 lattice
+// interpolation P with restriction P^T (ptatin_driver_linear_ts.c:987-996),
+// and an exact coarse solve.  Level 0 is the coarsest, as in PCMG.
+//
+// MI355X choices:
+//   * every vector stays in HBM for the whole solve; the host only sees the
+//     Krylov scalars;
+//   * the V-cycle runs in FP32 (operator, smoother, transfers) or FP64; the
+//     outer FGMRES, its true residual and the coarse solve are FP64;
+//   * restriction is written in gather form (one thread per coarse node), so
+//     neither transfer needs atomics;
+//   * each Chebyshev step is one MatMult plus one fused update kernel;
+//   * the coarsest level is factorised once (dense Cholesky, rocSOLVER).
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include "ptmf_internal.h"
+
+namespace ptmf {
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kRedBlocks = 1024;
+constexpr int kMaxDots = 64;
+
+inline unsigned nblocks(long n) {
+  long g = (n + kBS - 1) / kBS;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_axpby(long n, T a, const T *__restrict__ x, T b, T *__restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+
+template <typename T>
+__global__ void k_residual(long n, const T *__restrict__ b, const T *__restrict__ Ax, T *__restrict__ r) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    r[i] = b[i] - Ax[i];
+}
+
+template <typename Ts, typename Td>
+__global__ void k_cast(long n, const Ts *__restrict__ x, Td *__restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = (Td)x[i];
+}
+
+// y += (Td) x
+template <typename Ts, typename Td>
+__global__ void k_add_cast(long n, const Ts *__restrict__ x, Td *__restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] += (Td)x[i];
+}
+
+template <typename T>
+__global__ void k_recip(long n, const double *__restrict__ d, T *__restrict__ dinv) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dinv[i] = (T)(1.0 / d[i]);
+}
+
+// first Chebyshev step: p = x + scale * dinv * (b - Ax)   (Ax == nullptr: x = 0)
+template <typename T>
+__global__ void k_cheb_first(long n, const T *__restrict__ b, const T *__restrict__ Ax,
+                             const T *__restrict__ x, const T *__restrict__ dinv, T scale,
+                             T *__restrict__ p) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const T r = Ax ? b[i] - Ax[i] : b[i];
+    const T x0 = x ? x[i] : T(0);
+    p[i] = x0 + scale * (r * dinv[i]);
+  }
+}
+
+// p_kp1 = (omega scale) dinv (b - A p_k) + (1 - omega) p_km1 + omega p_k
+template <typename T>
+__global__ void k_cheb_step(long n, const T *__restrict__ b, const T *__restrict__ Apk,
+                            const T *__restrict__ dinv, const T *__restrict__ pkm1,
+                            const T *__restrict__ pk, T os, T om1, T om, T *__restrict__ pkp1) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const T z = (b[i] - Apk[i]) * dinv[i];
+    pkp1[i] = os * z + om1 * pkm1[i] + om * pk[i];
+  }
+}
+
+// partial sums of the dot products <V_j, w>, j in [j0, j0 + nk), nk <= 8,
+// one value per block: part[j * gridDim.x + block]
+__global__ void k_mdot_partial(long n, int j0, int nk, const double *const *__restrict__ V,
+                               const double *__restrict__ w, double *__restrict__ part) {
+  __shared__ double red[8][kBS / 64];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double wi = w[i];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (j < nk) acc[j] += V[j0 + j][i] * wi;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    double v = acc[j];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[j][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < nk) {
+    double t = 0.0;
+    for (int q = 0; q < kBS / 64; q++) t += red[threadIdx.x][q];
+    part[(long)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// partial sums of one dot product <a, b>
+__global__ void k_dot_partial(long n, const double *__restrict__ a, const double *__restrict__ b,
+                              double *__restrict__ part) {
+  __shared__ double red[kBS / 64];
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    acc += a[i] * b[i];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kBS / 64; q++) t += red[q];
+    part[blockIdx.x] = t;
+  }
+}
+
+// y = a x
+template <typename T>
+__global__ void k_scale(long n, T a, const T *__restrict__ x, T *__restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = a * x[i];
+}
+
+__global__ void k_mdot_final(int k, int nb, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double red[kBS];
+  for (int j = 0; j < k; j++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(long)j * nb + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
+    __syncthreads();
+  }
+}
+
+// w -= sum_j h_j V_j
+__global__ void k_maxpy(long n, int k, const double *const *__restrict__ V, const double *__restrict__ h,
+                        double *__restrict__ w) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double s = w[i];
+    for (int j = 0; j < k; j++) s -= h[j] * V[j][i];
+    w[i] = s;
+  }
+}
+
+// x += sum_j y_j Z_j
+__global__ void k_mupdate(long n, int k, const double *const *__restrict__ Z, const double *__restrict__ y,
+                          double *__restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double s = x[i];
+    for (int j = 0; j < k; j++) s += y[j] * Z[j][i];
+    x[i] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// transfers between level l-1 (coarse, mxc x myc x mzc) and l (fine, doubled)
+// ---------------------------------------------------------------------------
+// yf += P xc: fine node I takes coarse node I/2 (I even) or the mean of
+// coarse nodes (I-1)/2 and (I+1)/2 (I odd), tensor product, 3 components.
+template <typename T>
+__global__ void k_prolong_add(long Xc, long Yc, long Xf, long Yf, long Zf, const T *__restrict__ xc,
+                              T *__restrict__ yf) {
+  const long nf = Xf * Yf * Zf;
+  for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < nf; f += (long)gridDim.x * blockDim.x) {
+    const long I = f % Xf, J = (f / Xf) % Yf, K = f / (Xf * Yf);
+    long ci[2], cj[2], ck[2];
+    T wi[2], wj[2], wk[2];
+    int ni, nj, nk;
+    if (I & 1) { ci[0] = I >> 1; ci[1] = ci[0] + 1; wi[0] = wi[1] = T(0.5); ni = 2; } else { ci[0] = I >> 1; wi[0] = T(1); ni = 1; }
+    if (J & 1) { cj[0] = J >> 1; cj[1] = cj[0] + 1; wj[0] = wj[1] = T(0.5); nj = 2; } else { cj[0] = J >> 1; wj[0] = T(1); nj = 1; }
+    if (K & 1) { ck[0] = K >> 1; ck[1] = ck[0] + 1; wk[0] = wk[1] = T(0.5); nk = 2; } else { ck[0] = K >> 1; wk[0] = T(1); nk = 1; }
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int c = 0; c < nk; c++)
+      for (int b = 0; b < nj; b++)
+        for (int a = 0; a < ni; a++) {
+          const long g = ci[a] + Xc * (cj[b] + Yc * ck[c]);
+          const T w = wi[a] * wj[b] * wk[c];
+          s0 += w * xc[3 * g];
+          s1 += w * xc[3 * g + 1];
+          s2 += w * xc[3 * g + 2];
+        }
+    yf[3 * f] += s0;
+    yf[3 * f + 1] += s1;
+    yf[3 * f + 2] += s2;
+  }
+}
+
+// yc = P^T xf, gathered per coarse node from fine nodes 2i-1, 2i, 2i+1
+// (weights 1/2, 1, 1/2 per direction).
+template <typename T>
+__global__ void k_restrict(long Xc, long Yc, long Zc, long Xf, long Yf, long Zf, const T *__restrict__ xf,
+                           T *__restrict__ yc) {
+  const long ncn = Xc * Yc * Zc;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < ncn; g += (long)gridDim.x * blockDim.x) {
+    const long i = g % Xc, j = (g / Xc) % Yc, k = g / (Xc * Yc);
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int dk = -1; dk <= 1; dk++) {
+      const long K = 2 * k + dk;
+      if (K < 0 || K >= Zf) continue;
+      const T wk = dk == 0 ? T(1) : T(0.5);
+      for (int dj = -1; dj <= 1; dj++) {
+        const long J = 2 * j + dj;
+        if (J < 0 || J >= Yf) continue;
+        const T wj = dj == 0 ? T(1) : T(0.5);
+        for (int di = -1; di <= 1; di++) {
+          const long I = 2 * i + di;
+          if (I < 0 || I >= Xf) continue;
+          const T w = (di == 0 ? T(1) : T(0.5)) * wj * wk;
+          const long f = I + Xf * (J + Yf * K);
+          s0 += w * xf[3 * f];
+          s1 += w * xf[3 * f + 1];
+          s2 += w * xf[3 * f + 2];
+        }
+      }
+    }
+    yc[3 * g] = s0;
+    yc[3 * g + 1] = s1;
+    yc[3 * g + 2] = s2;
+  }
+}
+
+// coarse-element viscosity: mean of the 8 x 27 quadrature values of the
+// children (element-major input and output)
+__global__ void k_coarsen_eta(int mxc, int myc, int mzc, const double *__restrict__ ef,
+                              double *__restrict__ ec) {
+  const long nec = (long)mxc * myc * mzc;
+  const int mxf = 2 * mxc, myf = 2 * myc;
+  for (long E = blockIdx.x * (long)blockDim.x + threadIdx.x; E < nec; E += (long)gridDim.x * blockDim.x) {
+    const long ei = E % mxc, ej = (E / mxc) % myc, ek = E / ((long)mxc * myc);
+    double s = 0.0;
+    for (int c = 0; c < 2; c++)
+      for (int b = 0; b < 2; b++)
+        for (int a = 0; a < 2; a++) {
+          const long ef_e = (2 * ei + a) + (long)mxf * ((2 * ej + b) + (long)myf * (2 * ek + c));
+          for (int q = 0; q < 27; q++) s += ef[27 * ef_e + q];
+        }
+    s *= 1.0 / 216.0;
+    for (int q = 0; q < 27; q++) ec[27 * E + q] = s;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host-side multigrid object
+// ---------------------------------------------------------------------------
+struct MGLevel {
+  int mx = 0, my = 0, mz = 0;
+  ptmf_op op = nullptr;
+  long n = 0;  // 3 * nnodes
+  double *diag = nullptr, *dinv64 = nullptr;
+  float *dinv32 = nullptr;
+  double emin = 0, emax = 0;
+  bool have_eta = false, have_bc = false;
+  // work vectors
+  double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr, *p64[3] = {nullptr, nullptr, nullptr};
+  float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *p32[3] = {nullptr, nullptr, nullptr};
+};
+
+}  // namespace ptmf
+
+struct ptmf_mg_s {
+  int L = 0;
+  ptmf::MGLevel lev[16];
+  ptmf_mg_options opt;
+  bool is_setup = false;
+  hipStream_t stream = nullptr;
+  // reductions
+  double *red_part = nullptr, *red_out = nullptr, *red_host = nullptr;
+  const double **dV = nullptr, **dZ = nullptr;  // device arrays of Krylov vector pointers
+  double *coef_dev = nullptr, *coef_host = nullptr;  // Krylov coefficients (pinned upload)
+  // dense coarse factor
+  rocblas_handle blas = nullptr;
+  double *Adense = nullptr;
+  int ndense = 0;
+  int *dinfo = nullptr;
+  // Krylov workspace (finest level, FP64)
+  int restart = 0;
+  std::vector<double *> V, Z;
+  double *w = nullptr, *pcg[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<double> history;
+};
+
+namespace ptmf {
+namespace {
+
+int dalloc(void **p, size_t bytes) { return ptmf_malloc(p, bytes); }
+
+template <typename T>
+int axpby(hipStream_t s, long n, T a, const T *x, T b, T *y) {
+  k_axpby<T><<<nblocks(n), kBS, 0, s>>>(n, a, x, b, y);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+template <typename Ts, typename Td>
+int cast(hipStream_t s, long n, const Ts *x, Td *y) {
+  k_cast<Ts, Td><<<nblocks(n), kBS, 0, s>>>(n, x, y);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+// k dot products <V_j, w> (dV: device array of k vector pointers); results
+// on the host in out[0..k) and on the device in mg->red_out[0..k)
+int mdot(ptmf_mg_s *mg, long n, int k, const double *const *dV, const double *w, double *out) {
+  PTMF_CHECK(k > 0 && k <= kMaxDots, PTMF_ERR_ARG, "mdot: %d vectors", k);
+  for (int j0 = 0; j0 < k; j0 += 8) {
+    const int nk = k - j0 < 8 ? k - j0 : 8;
+    k_mdot_partial<<<kRedBlocks, kBS, 0, mg->stream>>>(n, j0, nk, dV, w, mg->red_part);
+    PTMF_HIP(hipGetLastError());
+  }
+  k_mdot_final<<<1, kBS, 0, mg->stream>>>(k, kRedBlocks, mg->red_part, mg->red_out);
+  PTMF_HIP(hipGetLastError());
+  PTMF_HIP(hipMemcpyAsync(mg->red_host, mg->red_out, sizeof(double) * k, hipMemcpyDeviceToHost,
+                          mg->stream));
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  memcpy(out, mg->red_host, sizeof(double) * k);
+  return PTMF_OK;
+}
+
+int dot(ptmf_mg_s *mg, long n, const double *a, const double *b, double *out) {
+  k_dot_partial<<<kRedBlocks, kBS, 0, mg->stream>>>(n, a, b, mg->red_part);
+  PTMF_HIP(hipGetLastError());
+  k_mdot_final<<<1, kBS, 0, mg->stream>>>(1, kRedBlocks, mg->red_part, mg->red_out);
+  PTMF_HIP(hipGetLastError());
+  PTMF_HIP(hipMemcpyAsync(mg->red_host, mg->red_out, sizeof(double), hipMemcpyDeviceToHost, mg->stream));
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  *out = mg->red_host[0];
+  return PTMF_OK;
+}
+
+template <typename T>
+int scale(hipStream_t s, long n, T a, const T *x, T *y) {
+  k_scale<T><<<nblocks(n), kBS, 0, s>>>(n, a, x, y);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+template <typename T>
+int matmult(const MGLevel &L, const T *x, T *y) {
+  return launch_a11_mult<T>(L.op, x, y, true);
+}
+
+// Chebyshev-Jacobi smoothing on level L (precision T): x <- cheb(b, x)
+template <typename T>
+int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
+  hipStream_t s = mg->stream;
+  const long n = L.n;
+  T *p[3], *r;
+  const T *dinv;
+  if (sizeof(T) == 8) {
+    p[0] = (T *)L.p64[0]; p[1] = (T *)L.p64[1]; p[2] = (T *)L.p64[2];
+    r = (T *)L.r64; dinv = (const T *)L.dinv64;
+  } else {
+    p[0] = (T *)L.p32[0]; p[1] = (T *)L.p32[1]; p[2] = (T *)L.p32[2];
+    r = (T *)L.r32; dinv = (const T *)L.dinv32;
+  }
+  const double emin = L.emin, emax = L.emax;
+  const double scale = 2.0 / (emax + emin);
+  const double alpha = 1.0 - scale * emin;
+  const double mu = 1.0 / alpha, omegaprod = 2.0 / alpha;
+  double c[3];
+  // p[km1] = x, p[k] = first step
+  int km1 = 0, k = 1, kp1 = 2;
+  if (nonzero)
+    PTMF_HIP(hipMemcpyAsync(p[km1], x, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+  else
+    PTMF_HIP(hipMemsetAsync(p[km1], 0, sizeof(T) * n, s));
+  if (nonzero) {
+    PTMF_TRY(matmult<T>(L, x, r));
+    k_cheb_first<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, x, dinv, (T)scale, p[k]);
+  } else {
+    k_cheb_first<T><<<nblocks(n), kBS, 0, s>>>(n, b, nullptr, nullptr, dinv, (T)scale, p[k]);
+  }
+  PTMF_HIP(hipGetLastError());
+  c[km1] = 1.0;
+  c[k] = mu;
+  for (int i = 0; i < its; i++) {
+    c[kp1] = 2.0 * mu * c[k] - c[km1];
+    const double omega = omegaprod * c[k] / c[kp1];
+    PTMF_TRY(matmult<T>(L, p[k], r));
+    k_cheb_step<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, dinv, p[km1], p[k], (T)(omega * scale),
+                                               (T)(1.0 - omega), (T)omega, p[kp1]);
+    PTMF_HIP(hipGetLastError());
+    const int t = km1;
+    km1 = k; k = kp1; kp1 = t;
+  }
+  PTMF_HIP(hipMemcpyAsync(x, p[k], sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+  return PTMF_OK;
+}
+
+// dense Cholesky solve on level 0: x = A0^-1 b (FP64)
+int dense_solve(ptmf_mg_s *mg, const double *b, double *x) {
+  const int n = mg->ndense;
+  PTMF_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+  rocblas_status st = rocsolver_dpotrs(mg->blas, rocblas_fill_lower, n, 1, mg->Adense, n, x, n);
+  PTMF_CHECK(st == rocblas_status_success, PTMF_ERR_DEVICE, "rocsolver_dpotrs: status %d", (int)st);
+  return PTMF_OK;
+}
+
+int vcycle64(ptmf_mg_s *mg, int l, int lc, const double *b, double *x);
+int coarse_solve(ptmf_mg_s *mg, int lc, const double *b, double *x);
+
+// V-cycle in precision T from level l down to the coarse-solve level lc;
+// x is overwritten (zero initial guess), b is the right-hand side.
+template <typename T>
+int vcycle(ptmf_mg_s *mg, int l, int lc, const T *b, T *x) {
+  MGLevel &L = mg->lev[l];
+  hipStream_t s = mg->stream;
+  if (l == lc) {
+    if (sizeof(T) == 8) return coarse_solve(mg, lc, (const double *)b, (double *)x);
+    // FP64 coarse solve of an FP32 V-cycle
+    PTMF_TRY((cast<float, double>(s, L.n, (const float *)b, L.b64)));
+    PTMF_TRY(coarse_solve(mg, lc, L.b64, L.x64));
+    return cast<double, float>(s, L.n, L.x64, (float *)x);
+  }
+  MGLevel &C = mg->lev[l - 1];
+  T *r = sizeof(T) == 8 ? (T *)L.r64 : (T *)L.r32;
+  T *bc = sizeof(T) == 8 ? (T *)C.b64 : (T *)C.b32;
+  T *xc = sizeof(T) == 8 ? (T *)C.x64 : (T *)C.x32;
+  // pre-smooth from zero
+  PTMF_TRY(cheb<T>(mg, L, b, x, mg->opt.pre_its, false));
+  // residual and restriction
+  PTMF_TRY(matmult<T>(L, x, r));
+  k_residual<T><<<nblocks(L.n), kBS, 0, s>>>(L.n, b, r, r);
+  PTMF_HIP(hipGetLastError());
+  const Box &bf = L.op->box, &bcx = C.op->box;
+  k_restrict<T><<<nblocks(bcx.nnodes), kBS, 0, s>>>(bcx.X, bcx.Y, bcx.Z, bf.X, bf.Y, bf.Z, r, bc);
+  PTMF_HIP(hipGetLastError());
+  PTMF_TRY(vcycle<T>(mg, l - 1, lc, bc, xc));
+  k_prolong_add<T><<<nblocks(bf.nnodes), kBS, 0, s>>>(bcx.X, bcx.Y, bf.X, bf.Y, bf.Z, xc, x);
+  PTMF_HIP(hipGetLastError());
+  // post-smooth from the corrected iterate
+  PTMF_TRY(cheb<T>(mg, L, b, x, mg->opt.post_its, true));
+  return PTMF_OK;
+}
+
+int vcycle64(ptmf_mg_s *mg, int l, int lc, const double *b, double *x) {
+  return vcycle<double>(mg, l, lc, b, x);
+}
+
+// Coarse solve on level lc: dense Cholesky on level 0, otherwise FP64 PCG
+// preconditioned by an FP64 V-cycle over levels lc..0.
+int coarse_solve(ptmf_mg_s *mg, int lc, const double *b, double *x) {
+  if (lc == 0) return dense_solve(mg, b, x);
+  MGLevel &L = mg->lev[lc];
+  hipStream_t s = mg->stream;
+  const long n = L.n;
+  double *r = mg->pcg[0], *z = mg->pcg[1], *p = mg->pcg[2], *q = mg->pcg[3];
+  PTMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+  PTMF_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  double rr0;
+  PTMF_TRY(dot(mg, n, r, r, &rr0));
+  if (rr0 == 0.0) return PTMF_OK;
+  // the inner V-cycle needs level lc's b/x buffers: use z as output
+  PTMF_TRY(vcycle64(mg, lc, 0, r, z));
+  PTMF_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  double rz;
+  PTMF_TRY(dot(mg, n, r, z, &rz));
+  for (int it = 0; it < mg->opt.coarse_maxit; it++) {
+    PTMF_TRY(matmult<double>(L, p, q));
+    double pq;
+    PTMF_TRY(dot(mg, n, p, q, &pq));
+    const double a = rz / pq;
+    PTMF_TRY(axpby<double>(s, n, a, p, 1.0, x));
+    PTMF_TRY(axpby<double>(s, n, -a, q, 1.0, r));
+    double rr;
+    PTMF_TRY(dot(mg, n, r, r, &rr));
+    PTMF_CHECK(isfinite(rr), PTMF_ERR_DIVERGED, "coarse PCG: non-finite residual");
+    if (sqrt(rr / rr0) < mg->opt.coarse_rtol) break;
+    PTMF_TRY(vcycle64(mg, lc, 0, r, z));
+    double rz1;
+    PTMF_TRY(dot(mg, n, r, z, &rz1));
+    const double beta = rz1 / rz;
+    rz = rz1;
+    PTMF_TRY(axpby<double>(s, n, 1.0, z, beta, p));
+  }
+  return PTMF_OK;
+}
+
+// largest eigenvalue of a symmetric tridiagonal matrix (bisection, Sturm count)
+double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &e) {
+  const int m = (int)d.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < m; i++) {
+    const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < m ? fabs(e[i]) : 0.0);
+    lo = fmin(lo, d[i] - r);
+    hi = fmax(hi, d[i] + r);
+  }
+  for (int it = 0; it < 200; it++) {
+    const double mid = 0.5 * (lo + hi);
+    int cnt = 0;  // eigenvalues < mid
+    double q = d[0] - mid;
+    if (q < 0) cnt++;
+    for (int i = 1; i < m; i++) {
+      if (q == 0.0) q = 1e-300;
+      q = d[i] - mid - e[i - 1] * e[i - 1] / q;
+      if (q < 0) cnt++;
+    }
+    if (cnt == m) hi = mid; else lo = mid;
+  }
+  return hi;
+}
+
+// Estimate the extreme eigenvalue of D^-1 A on level l with `its` steps of
+// Jacobi-preconditioned CG (Lanczos tridiagonal of the CG coefficients).
+int estimate_emax(ptmf_mg_s *mg, int l, int its, double *emax) {
+  MGLevel &L = mg->lev[l];
+  hipStream_t s = mg->stream;
+  const long n = L.n;
+  double *r = L.p64[0], *z = L.p64[1], *p = L.p64[2], *q = L.r64, *x = L.x64;
+  // seeded right-hand side on the host
+  std::vector<double> h(n);
+  unsigned long st = 0x9E3779B97F4A7C15UL ^ (unsigned long)n;
+  for (long i = 0; i < n; i++) {
+    st = st * 6364136223846793005UL + 1442695040888963407UL;
+    h[i] = ((double)(st >> 11) / (double)(1UL << 53)) - 0.5;
+  }
+  PTMF_HIP(hipMemcpyAsync(r, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  PTMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+  std::vector<double> alpha, beta;
+  // z = D^-1 r
+  auto jac = [&](const double *a, double *o) {
+    k_cheb_first<double><<<nblocks(n), kBS, 0, s>>>(n, a, nullptr, nullptr, L.dinv64, 1.0, o);
+  };
+  jac(r, z);
+  PTMF_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  double rz;
+  PTMF_TRY(dot(mg, n, r, z, &rz));
+  for (int it = 0; it < its; it++) {
+    PTMF_TRY(matmult<double>(L, p, q));
+    double pq;
+    PTMF_TRY(dot(mg, n, p, q, &pq));
+    if (!(pq > 0.0)) break;
+    const double a = rz / pq;
+    alpha.push_back(a);
+    PTMF_TRY(axpby<double>(s, n, -a, q, 1.0, r));
+    jac(r, z);
+    double rz1;
+    PTMF_TRY(dot(mg, n, r, z, &rz1));
+    const double b = rz1 / rz;
+    beta.push_back(b);
+    rz = rz1;
+    if (!(rz > 0.0)) break;
+    PTMF_TRY(axpby<double>(s, n, 1.0, z, b, p));
+  }
+  const int m = (int)alpha.size();
+  PTMF_CHECK(m > 0, PTMF_ERR_DIVERGED, "eigen estimate failed on level %d", l);
+  std::vector<double> d(m), e(m > 1 ? m - 1 : 1, 0.0);
+  for (int j = 0; j < m; j++) {
+    d[j] = 1.0 / alpha[j] + (j > 0 ? beta[j - 1] / alpha[j - 1] : 0.0);
+    if (j + 1 < m) e[j] = sqrt(beta[j]) / alpha[j];
+  }
+  *emax = tridiag_max_eig(d, e);
+  return PTMF_OK;
+}
+
+int assemble_dense(ptmf_mg_s *mg) {
+  MGLevel &L = mg->lev[0];
+  const long n = L.n;
+  PTMF_CHECK(n <= 6000, PTMF_ERR_UNSUPPORTED,
+             "coarsest level has %ld dofs; the dense coarse solve takes at most 6000", n);
+  mg->ndense = (int)n;
+  if (!mg->Adense) PTMF_TRY(dalloc((void **)&mg->Adense, sizeof(double) * n * n));
+  hipStream_t s = mg->stream;
+  std::vector<double> e(n, 0.0);
+  double *ej = L.p64[0];
+  for (long j = 0; j < n; j++) {
+    e[j] = 1.0;
+    PTMF_HIP(hipMemcpyAsync(ej, e.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    PTMF_TRY(matmult<double>(L, ej, mg->Adense + j * n));
+    PTMF_HIP(hipStreamSynchronize(s));
+    e[j] = 0.0;
+  }
+  if (!mg->dinfo) PTMF_TRY(dalloc((void **)&mg->dinfo, sizeof(int)));
+  rocblas_status st = rocsolver_dpotrf(mg->blas, rocblas_fill_lower, (int)n, mg->Adense, (int)n, mg->dinfo);
+  PTMF_CHECK(st == rocblas_status_success, PTMF_ERR_DEVICE, "rocsolver_dpotrf: status %d", (int)st);
+  int info = 0;
+  PTMF_HIP(hipMemcpyAsync(&info, mg->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  PTMF_HIP(hipStreamSynchronize(s));
+  PTMF_CHECK(info == 0, PTMF_ERR_DIVERGED, "coarse Cholesky failed (info %d): operator not SPD", info);
+  return PTMF_OK;
+}
+
+void set_op_stream(ptmf_op op, hipStream_t s) {
+  if (op->stream && op->stream != s) (void)hipStreamDestroy(op->stream);
+  op->stream = s;
+}
+
+}  // namespace
+}  // namespace ptmf
+
+using ptmf::MGLevel;
+
+extern "C" {
+
+int ptmf_mg_default_options(ptmf_mg_options *o) {
+  PTMF_CHECK(o, PTMF_ERR_ARG, "mg_default_options: null");
+  o->levels_mg = 0;
+  o->smoother_precision = 32;
+  o->pre_its = 3;
+  o->post_its = 3;
+  o->esteig[0] = 0.0;
+  o->esteig[1] = 0.2;
+  o->esteig[2] = 0.0;
+  o->esteig[3] = 1.1;
+  o->esteig_its = 20;
+  o->coarse_rtol = 1e-3;
+  o->coarse_maxit = 200;
+  return PTMF_OK;
+}
+
+int ptmf_mg_create(int mx, int my, int mz, int nlevels, ptmf_mg *out) {
+  PTMF_CHECK(out, PTMF_ERR_ARG, "mg_create: null");
+  *out = nullptr;
+  PTMF_CHECK(nlevels >= 1 && nlevels <= 16, PTMF_ERR_ARG, "mg_create: nlevels %d (1..16)", nlevels);
+  for (int l = 1; l < nlevels; l++) {
+    const int f = 1 << l;
+    PTMF_CHECK(mx % f == 0 && my % f == 0 && mz % f == 0, PTMF_ERR_ARG,
+               "mg_create: %d x %d x %d cannot be coarsened %d times by 2", mx, my, mz, nlevels - 1);
+  }
+  ptmf_mg_s *mg = new ptmf_mg_s();
+  ptmf_mg_default_options(&mg->opt);
+  mg->L = nlevels;
+  int st = PTMF_OK;
+  hipError_t he = hipStreamCreate(&mg->stream);
+  if (he != hipSuccess) st = ptmf_set_error(PTMF_ERR_DEVICE, "mg_create: %s", hipGetErrorString(he));
+  for (int l = 0; l < nlevels && st == PTMF_OK; l++) {
+    MGLevel &L = mg->lev[l];
+    const int f = 1 << (nlevels - 1 - l);
+    L.mx = mx / f; L.my = my / f; L.mz = mz / f;
+    st = ptmf_op_create(L.mx, L.my, L.mz, &L.op);
+    if (st != PTMF_OK) break;
+    ptmf::set_op_stream(L.op, mg->stream);
+    L.n = 3 * L.op->box.nnodes;
+    const size_t b8 = sizeof(double) * L.n, b4 = sizeof(float) * L.n;
+    void **a8[] = {(void **)&L.diag, (void **)&L.dinv64, (void **)&L.b64, (void **)&L.x64,
+                   (void **)&L.r64, (void **)&L.p64[0], (void **)&L.p64[1], (void **)&L.p64[2]};
+    for (void **p : a8)
+      if (st == PTMF_OK) st = ptmf::dalloc(p, b8);
+    void **a4[] = {(void **)&L.dinv32, (void **)&L.b32, (void **)&L.x32, (void **)&L.r32,
+                   (void **)&L.p32[0], (void **)&L.p32[1], (void **)&L.p32[2]};
+    for (void **p : a4)
+      if (st == PTMF_OK) st = ptmf::dalloc(p, b4);
+  }
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->red_part, sizeof(double) * ptmf::kRedBlocks * ptmf::kMaxDots);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->red_out, sizeof(double) * ptmf::kMaxDots);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->dV, sizeof(double *) * ptmf::kMaxDots);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->dZ, sizeof(double *) * ptmf::kMaxDots);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->coef_dev, sizeof(double) * ptmf::kMaxDots);
+  if (st == PTMF_OK) {
+    he = hipHostMalloc((void **)&mg->red_host, sizeof(double) * ptmf::kMaxDots, hipHostMallocDefault);
+    if (he == hipSuccess)
+      he = hipHostMalloc((void **)&mg->coef_host, sizeof(double) * ptmf::kMaxDots, hipHostMallocDefault);
+    if (he != hipSuccess) st = ptmf_set_error(PTMF_ERR_NOMEM, "mg_create: pinned buffer: %s", hipGetErrorString(he));
+  }
+  if (st == PTMF_OK) {
+    rocblas_status bs = rocblas_create_handle(&mg->blas);
+    if (bs != rocblas_status_success) st = ptmf_set_error(PTMF_ERR_DEVICE, "rocblas_create_handle: %d", (int)bs);
+    else rocblas_set_stream(mg->blas, mg->stream);
+  }
+  if (st != PTMF_OK) {
+    ptmf_mg tmp = mg;
+    ptmf_mg_destroy(&tmp);
+    return st;
+  }
+  *out = mg;
+  return PTMF_OK;
+}
+
+int ptmf_mg_destroy(ptmf_mg *pmg) {
+  if (!pmg || !*pmg) return PTMF_OK;
+  ptmf_mg_s *mg = *pmg;
+  if (mg->stream) (void)hipStreamSynchronize(mg->stream);
+  for (int l = 0; l < mg->L; l++) {
+    MGLevel &L = mg->lev[l];
+    if (L.op) {
+      L.op->stream = nullptr;  // shared stream, destroyed below
+      ptmf_op_destroy(&L.op);
+    }
+    void *ps[] = {L.diag, L.dinv64, L.b64, L.x64, L.r64, L.p64[0], L.p64[1], L.p64[2],
+                  L.dinv32, L.b32, L.x32, L.r32, L.p32[0], L.p32[1], L.p32[2]};
+    for (void *p : ps) (void)hipFree(p);
+  }
+  for (double *p : mg->V) (void)hipFree(p);
+  for (double *p : mg->Z) (void)hipFree(p);
+  (void)hipFree(mg->w);
+  for (double *p : mg->pcg) (void)hipFree(p);
+  (void)hipFree(mg->red_part);
+  (void)hipFree(mg->red_out);
+  (void)hipFree(mg->dV);
+  (void)hipFree(mg->dZ);
+  (void)hipFree(mg->coef_dev);
+  if (mg->coef_host) (void)hipHostFree(mg->coef_host);
+  (void)hipFree(mg->Adense);
+  (void)hipFree(mg->dinfo);
+  if (mg->red_host) (void)hipHostFree(mg->red_host);
+  if (mg->blas) rocblas_destroy_handle(mg->blas);
+  if (mg->stream) (void)hipStreamDestroy(mg->stream);
+  delete mg;
+  *pmg = nullptr;
+  return PTMF_OK;
+}
+
+int ptmf_mg_levels(ptmf_mg mg, int *nlevels) {
+  PTMF_CHECK(mg && nlevels, PTMF_ERR_ARG, "mg_levels: null");
+  *nlevels = mg->L;
+  return PTMF_OK;
+}
+
+int ptmf_mg_level_op(ptmf_mg mg, int level, ptmf_op *op) {
+  PTMF_CHECK(mg && op && level >= 0 && level < mg->L, PTMF_ERR_ARG, "mg_level_op: bad level %d", level);
+  *op = mg->lev[level].op;
+  return PTMF_OK;
+}
+
+int ptmf_mg_set_coordinates(ptmf_mg mg, const double *coords) {
+  PTMF_CHECK(mg && coords, PTMF_ERR_ARG, "mg_set_coordinates: null");
+  // finest level takes the input; coarser levels by injection
+  // (DMDARestrictCoordinatesHierarchy, dmda_project_coords.c:39-71)
+  std::vector<double> cur(coords, coords + 3 * mg->lev[mg->L - 1].op->box.nnodes);
+  for (int l = mg->L - 1; l >= 0; l--) {
+    MGLevel &L = mg->lev[l];
+    PTMF_TRY(ptmf_op_set_coordinates(L.op, cur.data()));
+    if (l == 0) break;
+    const ptmf::Box &bf = L.op->box, &bc = mg->lev[l - 1].op->box;
+    std::vector<double> nxt(3 * bc.nnodes);
+    for (long k = 0; k < bc.Z; k++)
+      for (long j = 0; j < bc.Y; j++)
+        for (long i = 0; i < bc.X; i++) {
+          const long g = i + bc.X * (j + bc.Y * k), f = 2 * i + bf.X * (2 * j + bf.Y * 2 * k);
+          for (int d = 0; d < 3; d++) nxt[3 * g + d] = cur[3 * f + d];
+        }
+    cur.swap(nxt);
+  }
+  mg->is_setup = false;
+  return PTMF_OK;
+}
+
+int ptmf_mg_set_viscosity(ptmf_mg mg, int level, const double *eta) {
+  PTMF_CHECK(mg && eta && level >= 0 && level < mg->L, PTMF_ERR_ARG, "mg_set_viscosity: bad level %d", level);
+  PTMF_TRY(ptmf_op_set_viscosity(mg->lev[level].op, eta));
+  mg->lev[level].have_eta = true;
+  mg->is_setup = false;
+  return PTMF_OK;
+}
+
+int ptmf_mg_coarsen_viscosity(ptmf_mg mg) {
+  PTMF_CHECK(mg, PTMF_ERR_ARG, "mg_coarsen_viscosity: null");
+  PTMF_CHECK(mg->lev[mg->L - 1].have_eta, PTMF_ERR_NOT_SETUP, "mg_coarsen_viscosity: finest viscosity not set");
+  for (int l = mg->L - 1; l > 0; l--) {
+    MGLevel &F = mg->lev[l], &Cl = mg->lev[l - 1];
+    const long nef = F.op->box.nel, nec = Cl.op->box.nel;
+    double *ef = nullptr, *ec = nullptr;
+    PTMF_TRY(ptmf::dalloc((void **)&ef, sizeof(double) * 27 * nef));
+    PTMF_TRY(ptmf::dalloc((void **)&ec, sizeof(double) * 27 * nec));
+    // the operator stores eta qp-major; rebuild element-major on the host
+    std::vector<double> hq(27 * nef), he(27 * nef);
+    PTMF_HIP(hipMemcpy(hq.data(), F.op->d64.eta, sizeof(double) * 27 * nef, hipMemcpyDeviceToHost));
+    for (long e = 0; e < nef; e++)
+      for (int q = 0; q < 27; q++) he[27 * e + q] = hq[(long)q * nef + e];
+    PTMF_HIP(hipMemcpy(ef, he.data(), sizeof(double) * 27 * nef, hipMemcpyHostToDevice));
+    ptmf::k_coarsen_eta<<<ptmf::nblocks(nec), ptmf::kBS, 0, mg->stream>>>(Cl.mx, Cl.my, Cl.mz, ef, ec);
+    PTMF_HIP(hipGetLastError());
+    std::vector<double> hc(27 * nec);
+    PTMF_HIP(hipMemcpyAsync(hc.data(), ec, sizeof(double) * 27 * nec, hipMemcpyDeviceToHost, mg->stream));
+    PTMF_HIP(hipStreamSynchronize(mg->stream));
+    (void)hipFree(ef);
+    (void)hipFree(ec);
+    PTMF_TRY(ptmf_op_set_viscosity(Cl.op, hc.data()));
+    Cl.have_eta = true;
+  }
+  mg->is_setup = false;
+  return PTMF_OK;
+}
+
+int ptmf_mg_set_dirichlet(ptmf_mg mg, int level, const unsigned char *mask) {
+  PTMF_CHECK(mg && mask && level >= 0 && level < mg->L, PTMF_ERR_ARG, "mg_set_dirichlet: bad level %d", level);
+  PTMF_TRY(ptmf_op_set_dirichlet(mg->lev[level].op, mask));
+  mg->lev[level].have_bc = true;
+  mg->is_setup = false;
+  return PTMF_OK;
+}
+
+int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt) {
+  PTMF_CHECK(mg, PTMF_ERR_ARG, "mg_setup: null");
+  if (opt) mg->opt = *opt;
+  ptmf_mg_options &o = mg->opt;
+  if (o.levels_mg <= 0 || o.levels_mg > mg->L) o.levels_mg = mg->L;
+  PTMF_CHECK(o.smoother_precision == 32 || o.smoother_precision == 64, PTMF_ERR_UNSUPPORTED,
+             "mg_setup: smoother precision %d (32 or 64)", o.smoother_precision);
+  PTMF_CHECK(o.pre_its >= 0 && o.post_its >= 0 && o.esteig_its > 0, PTMF_ERR_ARG, "mg_setup: bad iteration counts");
+  hipStream_t s = mg->stream;
+  for (int l = 0; l < mg->L; l++) {
+    MGLevel &L = mg->lev[l];
+    PTMF_CHECK(L.op->has_coords && L.have_eta && L.have_bc, PTMF_ERR_NOT_SETUP,
+               "mg_setup: level %d lacks coordinates, viscosity or Dirichlet list", l);
+    PTMF_TRY(ptmf_a11_diagonal(L.op, L.diag));
+    ptmf::k_recip<double><<<ptmf::nblocks(L.n), ptmf::kBS, 0, s>>>(L.n, L.diag, L.dinv64);
+    ptmf::k_recip<float><<<ptmf::nblocks(L.n), ptmf::kBS, 0, s>>>(L.n, L.diag, L.dinv32);
+    PTMF_HIP(hipGetLastError());
+    if (l > 0) {
+      double est = 0.0;
+      PTMF_TRY(ptmf::estimate_emax(mg, l, o.esteig_its, &est));
+      // KSPChebyshevEstEigSet(ksp, a, b, c, d): emin = a*min + b*max, emax = c*min + d*max;
+      // with the default transform only the max estimate enters
+      L.emin = o.esteig[0] * est + o.esteig[1] * est;
+      L.emax = o.esteig[2] * est + o.esteig[3] * est;
+    }
+  }
+  PTMF_TRY(ptmf::assemble_dense(mg));
+  // coarse PCG workspace (level levels_mg coarse index)
+  const int lc = mg->L - o.levels_mg;
+  if (lc > 0) {
+    const long n = mg->lev[lc].n;
+    for (int i = 0; i < 4; i++) {
+      if (mg->pcg[i]) (void)hipFree(mg->pcg[i]);
+      PTMF_TRY(ptmf::dalloc((void **)&mg->pcg[i], sizeof(double) * n));
+    }
+  }
+  PTMF_HIP(hipStreamSynchronize(s));
+  mg->is_setup = true;
+  return PTMF_OK;
+}
+
+int ptmf_mg_level_eigs(ptmf_mg mg, int level, double *emin, double *emax) {
+  PTMF_CHECK(mg && level >= 0 && level < mg->L, PTMF_ERR_ARG, "mg_level_eigs: bad level");
+  if (emin) *emin = mg->lev[level].emin;
+  if (emax) *emax = mg->lev[level].emax;
+  return PTMF_OK;
+}
+
+int ptmf_mg_set_level_eigs(ptmf_mg mg, int level, double emin, double emax) {
+  PTMF_CHECK(mg && level >= 0 && level < mg->L && emax > emin && emin > 0, PTMF_ERR_ARG,
+             "mg_set_level_eigs: bad arguments");
+  mg->lev[level].emin = emin;
+  mg->lev[level].emax = emax;
+  return PTMF_OK;
+}
+
+// x = V-cycle(b) on the finest level (FP64 in and out)
+int ptmf_mg_vcycle(ptmf_mg mg, const double *b, double *x) {
+  PTMF_CHECK(mg && b && x, PTMF_ERR_ARG, "mg_vcycle: null");
+  PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_vcycle: not set up");
+  const int lf = mg->L - 1, lc = mg->L - mg->opt.levels_mg;
+  MGLevel &F = mg->lev[lf];
+  if (mg->opt.smoother_precision == 64 || lf == lc) {
+    PTMF_TRY(ptmf::vcycle<double>(mg, lf, lc, b, x));
+  } else {
+    PTMF_TRY((ptmf::cast<double, float>(mg->stream, F.n, b, F.b32)));
+    PTMF_TRY(ptmf::vcycle<float>(mg, lf, lc, F.b32, F.x32));
+    PTMF_TRY((ptmf::cast<float, double>(mg->stream, F.n, F.x32, x)));
+  }
+  return PTMF_OK;
+}
+
+int ptmf_mg_prolong_add(ptmf_mg mg, int level, const double *xc, double *yf) {
+  PTMF_CHECK(mg && xc && yf && level >= 1 && level < mg->L, PTMF_ERR_ARG, "mg_prolong_add: bad level %d", level);
+  const ptmf::Box &bf = mg->lev[level].op->box, &bc = mg->lev[level - 1].op->box;
+  ptmf::k_prolong_add<double><<<ptmf::nblocks(bf.nnodes), ptmf::kBS, 0, mg->stream>>>(bc.X, bc.Y, bf.X, bf.Y, bf.Z, xc, yf);
+  PTMF_HIP(hipGetLastError());
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  return PTMF_OK;
+}
+
+int ptmf_mg_restrict(ptmf_mg mg, int level, const double *xf, double *yc) {
+  PTMF_CHECK(mg && xf && yc && level >= 1 && level < mg->L, PTMF_ERR_ARG, "mg_restrict: bad level %d", level);
+  const ptmf::Box &bf = mg->lev[level].op->box, &bc = mg->lev[level - 1].op->box;
+  ptmf::k_restrict<double><<<ptmf::nblocks(bc.nnodes), ptmf::kBS, 0, mg->stream>>>(bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z, xf, yc);
+  PTMF_HIP(hipGetLastError());
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  return PTMF_OK;
+}
+
+int ptmf_mg_smooth(ptmf_mg mg, int level, int precision, int its, int nonzero_guess, const void *b, void *x) {
+  PTMF_CHECK(mg && b && x && level >= 0 && level < mg->L, PTMF_ERR_ARG, "mg_smooth: bad level %d", level);
+  PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_smooth: not set up");
+  MGLevel &L = mg->lev[level];
+  PTMF_CHECK(L.emax > 0, PTMF_ERR_NOT_SETUP, "mg_smooth: level %d has no eigenvalue bounds", level);
+  if (precision == 64)
+    PTMF_TRY(ptmf::cheb<double>(mg, L, (const double *)b, (double *)x, its, nonzero_guess != 0));
+  else if (precision == 32)
+    PTMF_TRY(ptmf::cheb<float>(mg, L, (const float *)b, (float *)x, its, nonzero_guess != 0));
+  else
+    return ptmf_set_error(PTMF_ERR_UNSUPPORTED, "mg_smooth: precision %d", precision);
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  return PTMF_OK;
+}
+
+// Right-preconditioned FGMRES(restart) on the finest-level A11 (FP64) with the
+// V-cycle as (variable) preconditioner.  Converged when the FGMRES residual
+// drops below rtol * ||b||; the true FP64 residual is recomputed at every
+// restart and at the end.
+int ptmf_mg_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int restart,
+                   int *its_out, double *rnorm_out, double *history, int history_len) {
+  PTMF_CHECK(mg && b && x && restart > 0 && restart < ptmf::kMaxDots && maxit > 0, PTMF_ERR_ARG,
+             "mg_fgmres: bad arguments");
+  PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_fgmres: not set up");
+  MGLevel &F = mg->lev[mg->L - 1];
+  const long n = F.n;
+  hipStream_t s = mg->stream;
+  if (mg->restart != restart) {
+    PTMF_HIP(hipStreamSynchronize(s));
+    for (double *p : mg->V) (void)hipFree(p);
+    for (double *p : mg->Z) (void)hipFree(p);
+    mg->V.assign(restart + 1, nullptr);
+    mg->Z.assign(restart, nullptr);
+    for (auto &p : mg->V) PTMF_TRY(ptmf::dalloc((void **)&p, sizeof(double) * n));
+    for (auto &p : mg->Z) PTMF_TRY(ptmf::dalloc((void **)&p, sizeof(double) * n));
+    if (!mg->w) PTMF_TRY(ptmf::dalloc((void **)&mg->w, sizeof(double) * n));
+    PTMF_HIP(hipMemcpy(mg->dV, mg->V.data(), sizeof(double *) * (restart + 1), hipMemcpyHostToDevice));
+    PTMF_HIP(hipMemcpy(mg->dZ, mg->Z.data(), sizeof(double *) * restart, hipMemcpyHostToDevice));
+    mg->restart = restart;
+  }
+  double *w = mg->w;
+  std::vector<double> H((restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart);
+  double bnorm2;
+  PTMF_TRY(ptmf::dot(mg, n, b, b, &bnorm2));
+  const double bnorm = sqrt(bnorm2);
+  int its = 0, hl = 0;
+  double rnorm = 0.0;
+  if (bnorm == 0.0) {
+    PTMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    PTMF_HIP(hipStreamSynchronize(s));
+    if (its_out) *its_out = 0;
+    if (rnorm_out) *rnorm_out = 0.0;
+    return PTMF_OK;
+  }
+  while (true) {
+    // true residual r = b - A x into V0
+    PTMF_TRY(ptmf::matmult<double>(F, x, w));
+    ptmf::k_residual<double><<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, b, w, mg->V[0]);
+    PTMF_HIP(hipGetLastError());
+    double rr;
+    PTMF_TRY(ptmf::dot(mg, n, mg->V[0], mg->V[0], &rr));
+    rnorm = sqrt(rr);
+    PTMF_CHECK(isfinite(rnorm), PTMF_ERR_DIVERGED, "fgmres: non-finite residual");
+    if (history && hl < history_len) history[hl++] = rnorm;
+    if (rnorm <= rtol * bnorm || its >= maxit) break;
+    PTMF_TRY(ptmf::scale<double>(s, n, 1.0 / rnorm, mg->V[0], mg->V[0]));
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = rnorm;
+    int j = 0;
+    while (j < restart && its < maxit) {
+      PTMF_TRY(ptmf_mg_vcycle(mg, mg->V[j], mg->Z[j]));
+      PTMF_TRY(ptmf::matmult<double>(F, mg->Z[j], w));
+      // classical Gram-Schmidt, applied twice; the dots stay on the device
+      double h[ptmf::kMaxDots], h2[ptmf::kMaxDots];
+      PTMF_TRY(ptmf::mdot(mg, n, j + 1, mg->dV, w, h));
+      ptmf::k_maxpy<<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, j + 1, mg->dV, mg->red_out, w);
+      PTMF_HIP(hipGetLastError());
+      PTMF_TRY(ptmf::mdot(mg, n, j + 1, mg->dV, w, h2));
+      ptmf::k_maxpy<<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, j + 1, mg->dV, mg->red_out, w);
+      PTMF_HIP(hipGetLastError());
+      for (int i = 0; i <= j; i++) h[i] += h2[i];
+      double ww;
+      PTMF_TRY(ptmf::dot(mg, n, w, w, &ww));
+      const double hn = sqrt(ww);
+      PTMF_CHECK(isfinite(hn), PTMF_ERR_DIVERGED, "fgmres: non-finite Arnoldi vector");
+      for (int i = 0; i < j; i++) {
+        const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+        h[i] = t;
+      }
+      const double den = sqrt(h[j] * h[j] + hn * hn);
+      cs[j] = den > 0 ? h[j] / den : 1.0;
+      sn[j] = den > 0 ? hn / den : 0.0;
+      h[j] = den;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int i = 0; i <= j; i++) H[i * restart + j] = h[i];
+      its++;
+      j++;
+      if (history && hl < history_len) history[hl++] = fabs(g[j]);
+      if (fabs(g[j]) <= rtol * bnorm || hn == 0.0) break;
+      PTMF_TRY(ptmf::scale<double>(s, n, 1.0 / hn, w, mg->V[j]));
+    }
+    // solve the triangular system and update x += Z y
+    for (int i = j - 1; i >= 0; i--) {
+      double t = g[i];
+      for (int k = i + 1; k < j; k++) t -= H[i * restart + k] * y[k];
+      y[i] = t / H[i * restart + i];
+    }
+    memcpy(mg->coef_host, y.data(), sizeof(double) * j);
+    PTMF_HIP(hipMemcpyAsync(mg->coef_dev, mg->coef_host, sizeof(double) * j, hipMemcpyHostToDevice, s));
+    ptmf::k_mupdate<<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, j, mg->dZ, mg->coef_dev, x);
+    PTMF_HIP(hipGetLastError());
+  }
+  PTMF_HIP(hipStreamSynchronize(s));
+  if (its_out) *its_out = its;
+  if (rnorm_out) *rnorm_out = rnorm;
+  return PTMF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+"""Host-side mirror of the reference's GMG-preconditioned viscous solve.
+
+`GMG` holds a level hierarchy of matrix-free A11 operators (level 0 =
+coarsest, as PETSc PCMG) on the device, configured like the reference's
+options (pTatin3dStokesKSPConfigureFSGMG, ptatin_driver_linear_ts.c:819-882):
+Chebyshev/Jacobi smoothing with eigenvalue bounds from an estimate
+(-..._esteig a,b,c,d), transfers from DMDA Q1 lattice interpolation, an exact
+coarse solve and FGMRES outside.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib, fields as F
+from ._lib import DeviceArray, MGOptions, call
+
+
+def default_options(**kw) -> MGOptions:
+    o = MGOptions()
+    call("ptmf_mg_default_options", C.byref(o))
+    for k, v in kw.items():
+        if k == "esteig":
+            for i in range(4):
+                o.esteig[i] = float(v[i])
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class GMG:
+    def __init__(self, mx: int, my: int, mz: int, nlevels: int):
+        h = C.c_void_p()
+        call("ptmf_mg_create", int(mx), int(my), int(mz), int(nlevels), C.byref(h))
+        self.handle = h
+        self.nlevels = nlevels
+        self.dims = [(mx >> (nlevels - 1 - l), my >> (nlevels - 1 - l), mz >> (nlevels - 1 - l))
+                     for l in range(nlevels)]
+        self.n = 3 * F.nnodes(mx, my, mz)
+
+    def level_coordinates(self, fine_coords: np.ndarray) -> list[np.ndarray]:
+        """Injected coordinates of every level (for host-side field evaluation)."""
+        out = [None] * self.nlevels
+        cur = np.asarray(fine_coords, dtype=np.float64).reshape(-1, 3)
+        for l in range(self.nlevels - 1, -1, -1):
+            mx, my, mz = self.dims[l]
+            out[l] = cur.reshape(-1)
+            if l == 0:
+                break
+            X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+            c3 = cur.reshape(Z, Y, X, 3)
+            cur = np.ascontiguousarray(c3[::2, ::2, ::2]).reshape(-1, 3)
+        return out
+
+    def set_coordinates(self, coords: np.ndarray) -> None:
+        c = np.ascontiguousarray(coords, dtype=np.float64)
+        call("ptmf_mg_set_coordinates", self.handle, _lib._hp(c))
+
+    def set_viscosity(self, level: int, eta: np.ndarray) -> None:
+        e = np.ascontiguousarray(eta, dtype=np.float64)
+        call("ptmf_mg_set_viscosity", self.handle, int(level), _lib._hp(e))
+
+    def coarsen_viscosity(self) -> None:
+        call("ptmf_mg_coarsen_viscosity", self.handle)
+
+    def set_dirichlet(self, level: int, mask: np.ndarray) -> None:
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        call("ptmf_mg_set_dirichlet", self.handle, int(level), _lib._hp(m))
+
+    def set_dirichlet_all_faces(self) -> None:
+        for l, (mx, my, mz) in enumerate(self.dims):
+            self.set_dirichlet(l, F.dirichlet_all_faces(mx, my, mz))
+
+    def setup(self, opt: MGOptions | None = None) -> None:
+        call("ptmf_mg_setup", self.handle, C.byref(opt) if opt is not None else None)
+
+    def eigs(self, level: int) -> tuple[float, float]:
+        a, b = C.c_double(), C.c_double()
+        call("ptmf_mg_level_eigs", self.handle, int(level), C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def set_eigs(self, level: int, emin: float, emax: float) -> None:
+        call("ptmf_mg_set_level_eigs", self.handle, int(level), float(emin), float(emax))
+
+    def level_op_handle(self, level: int) -> C.c_void_p:
+        h = C.c_void_p()
+        call("ptmf_mg_level_op", self.handle, int(level), C.byref(h))
+        return h
+
+    def vcycle(self, b: DeviceArray, x: DeviceArray) -> None:
+        call("ptmf_mg_vcycle", self.handle, b.ptr, x.ptr)
+
+    def prolong_add(self, level: int, xc: DeviceArray, yf: DeviceArray) -> None:
+        call("ptmf_mg_prolong_add", self.handle, int(level), xc.ptr, yf.ptr)
+
+    def restrict(self, level: int, xf: DeviceArray, yc: DeviceArray) -> None:
+        call("ptmf_mg_restrict", self.handle, int(level), xf.ptr, yc.ptr)
+
+    def smooth(self, level: int, b: DeviceArray, x: DeviceArray, its: int, nonzero_guess: bool) -> None:
+        prec = 32 if b.dtype == np.float32 else 64
+        call("ptmf_mg_smooth", self.handle, int(level), prec, int(its), int(nonzero_guess), b.ptr, x.ptr)
+
+    def fgmres(self, b: DeviceArray, x: DeviceArray, rtol=1e-8, maxit=200, restart=30,
+               history_len=512):
+        its = C.c_int(0)
+        rn = C.c_double(0)
+        hist = np.zeros(history_len)
+        call("ptmf_mg_fgmres", self.handle, b.ptr, x.ptr, float(rtol), int(maxit), int(restart),
+             C.byref(its), C.byref(rn), hist.ctypes.data_as(C.POINTER(C.c_double)), int(history_len))
+        nz = np.flatnonzero(hist)
+        return its.value, rn.value, hist[: (nz[-1] + 1) if nz.size else 0]
+
+    def destroy(self) -> None:
+        if self.handle is not None and self.handle.value:
+            call("ptmf_mg_destroy", C.byref(self.handle))
+        self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle is not None and self.handle.value and _lib._lib is not None:
+                _lib._lib.ptmf_mg_destroy(C.byref(self.handle))
+        except Exception:
+            pass
+
+
+def solve_timed(mg: GMG, b: np.ndarray, rtol=1e-8, maxit=200, restart=30):
+    """Upload b, run FGMRES from x = 0; returns (x, its, rnorm, history, seconds)."""
+    db = DeviceArray.from_host(b)
+    dx = DeviceArray(b.size)
+    dx.zero()
+    call("ptmf_device_synchronize")
+    t0 = time.perf_counter()
+    its, rn, hist = mg.fgmres(db, dx, rtol, maxit, restart)
+    call("ptmf_device_synchronize")
+    t1 = time.perf_counter()
+    x = dx.download()
+    db.free()
+    dx.free()
+    return x, its, rn, hist, t1 - t0

@@ -1,0 +1,130 @@
+"""GPU tests of the multigrid pieces against the CPU oracle, through the C-ABI.
+
+Transfers and the FP64 Chebyshev-Jacobi smoother are compared with the
+oracle at 1e-12 relative; solves are checked through their true FP64
+residual, recomputed by the oracle operator on the host.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ptatin3d_amd import fields as F
+from ptatin3d_amd._lib import DeviceArray
+from ptatin3d_amd.multigrid import GMG, default_options, solve_timed
+
+from conftest import rel_inf
+
+pytestmark = pytest.mark.gpu
+
+
+def _hierarchy(m, nlevels, perturb=0.1, lo=1e-2, hi=1e2, seed=3, opt=None):
+    rng = np.random.default_rng(seed)
+    mg = GMG(*m, nlevels)
+    coords = F.box_coords(*m)
+    if perturb:
+        coords = F.perturb_interior(coords, *m, perturb, rng)
+    mg.set_coordinates(coords)
+    eta = F.eta_loguniform(m[0] * m[1] * m[2], lo, hi, rng)
+    mg.set_viscosity(nlevels - 1, eta)
+    mg.coarsen_viscosity()
+    mg.set_dirichlet_all_faces()
+    mg.setup(opt or default_options())
+    return mg, coords, eta
+
+
+def test_transfers_match_oracle(gpu):
+    m = (6, 4, 8)
+    mg, coords, _ = _hierarchy(m, 2)
+    mc = (3, 2, 4)
+    rng = np.random.default_rng(1)
+    xc = rng.uniform(-1, 1, 3 * F.nnodes(*mc))
+    yf0 = rng.uniform(-1, 1, 3 * F.nnodes(*m))
+    dxc, dyf = DeviceArray.from_host(xc), DeviceArray.from_host(yf0)
+    mg.prolong_add(1, dxc, dyf)
+    assert rel_inf(dyf.download(), O.prolong_add(*mc, xc, yf0.copy())) <= 1e-14
+    xf = rng.uniform(-1, 1, 3 * F.nnodes(*m))
+    dxf, dyc = DeviceArray.from_host(xf), DeviceArray(3 * F.nnodes(*mc))
+    mg.restrict(1, dxf, dyc)
+    assert rel_inf(dyc.download(), O.restrict_add(*mc, xf, np.zeros(3 * F.nnodes(*mc)))) <= 1e-14
+    mg.destroy()
+
+
+def test_chebyshev_matches_oracle(gpu):
+    m = (4, 6, 4)
+    mg, coords, eta = _hierarchy(m, 2)
+    bc = F.dirichlet_all_faces(*m)
+    d = O.a11_diagonal(*m, coords, eta, bc)
+    emin, emax = 0.3, 3.1
+    mg.set_eigs(1, emin, emax)
+    rng = np.random.default_rng(5)
+    b = rng.uniform(-1, 1, 3 * F.nnodes(*m))
+    x0 = rng.uniform(-1, 1, b.size)
+    for nonzero in (False, True):
+        db = DeviceArray.from_host(b)
+        dx = DeviceArray.from_host(x0 if nonzero else np.zeros_like(x0))
+        mg.smooth(1, db, dx, 4, nonzero)
+        xo = O.cheb_jacobi(*m, coords, eta, bc, d, emin, emax, 4, b, x0 if nonzero else None)
+        assert rel_inf(dx.download(), xo) <= 1e-12
+    mg.destroy()
+
+
+def test_eigen_estimate_bounds_spectrum(gpu):
+    """The Lanczos estimate of lambda_max(D^-1 A) is close to and not above the
+    true value (dense eigen-solve of the oracle operator)."""
+    m = (2, 2, 2)
+    mg, coords, eta = _hierarchy(m, 2, seed=8)
+    bc = F.dirichlet_all_faces(*m)
+    n = 3 * F.nnodes(*m)
+    A = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        A[:, i] = O.a11_matmult(*m, coords, eta, bc, e)
+    d = np.diag(A)
+    lam = np.linalg.eigvalsh(A / np.sqrt(np.outer(d, d)))
+    emin, emax = mg.eigs(1)
+    est = emax / 1.1
+    assert 0.9 * lam[-1] <= est <= lam[-1] * (1 + 1e-8)
+    assert abs(emin - 0.2 * est) <= 1e-12 * est
+    mg.destroy()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fgmres_gmg_solve(gpu, prec):
+    m = (16, 16, 16)
+    opt = default_options(smoother_precision=prec, pre_its=3, post_its=3)
+    mg, coords, eta = _hierarchy(m, 4, opt=opt)
+    bc = F.dirichlet_all_faces(*m)
+    rng = np.random.default_rng(2)
+    b = rng.uniform(-1, 1, 3 * F.nnodes(*m))
+    b[bc.astype(bool)] = 0.0
+    x, its, rn, hist, secs = solve_timed(mg, b, rtol=1e-8, maxit=200, restart=30)
+    r = b - O.a11_matmult(*m, coords, eta, bc, x)
+    assert np.linalg.norm(r) <= 1.5e-8 * np.linalg.norm(b), (its, np.linalg.norm(r) / np.linalg.norm(b))
+    assert its < 60, its
+    mg.destroy()
+
+
+def test_dense_coarse_only(gpu):
+    """A one-level hierarchy is the exact (Cholesky) solve: one iteration."""
+    m = (2, 2, 2)
+    mg, coords, eta = _hierarchy(m, 1)
+    bc = F.dirichlet_all_faces(*m)
+    b = np.random.default_rng(4).uniform(-1, 1, 3 * F.nnodes(*m))
+    x, its, rn, hist, secs = solve_timed(mg, b, rtol=1e-12, maxit=5)
+    assert its <= 1
+    assert rel_inf(O.a11_matmult(*m, coords, eta, bc, x), b) <= 1e-11
+    mg.destroy()
+
+
+def test_pcg_coarse_solver(gpu):
+    """levels_mg < nlevels: the coarse level is solved by FP64 PCG + V-cycle."""
+    m = (8, 8, 8)
+    opt = default_options(levels_mg=2, coarse_rtol=1e-6)
+    mg, coords, eta = _hierarchy(m, 4, opt=opt)
+    bc = F.dirichlet_all_faces(*m)
+    b = np.random.default_rng(6).uniform(-1, 1, 3 * F.nnodes(*m))
+    x, its, rn, hist, secs = solve_timed(mg, b, rtol=1e-8, maxit=100)
+    r = b - O.a11_matmult(*m, coords, eta, bc, x)
+    assert np.linalg.norm(r) <= 1.5e-8 * np.linalg.norm(b)
+    mg.destroy()
