@@ -68,6 +68,13 @@ typedef struct {
   int esteig_its;         /* Lanczos (Jacobi-PCG) steps of the eigenvalue estimate */
   double coarse_rtol;     /* relative tolerance of the coarse PCG */
   int coarse_maxit;
+  int galerkin_levels;    /* coarsest levels whose operator is the Galerkin product
+                             R A_{l+1} P (OP_TYPE_GALERKIN of -A11_operator_type,
+                             ptatin_driver_linear_ts.c:712-793) instead of the
+                             rediscretised matrix-free operator; 0: none */
+  int bc_transfer;        /* 1: the coarse correction vanishes on Dirichlet dofs (the
+                             restricted residual is zeroed there); 0: plain R = P^T as
+                             PCMG with DMDA interpolation (ptatin_driver_linear_ts.c:849) */
 } ptmf_mg_options;
 
 /* ---- library / device helpers ------------------------------------------ */
@@ -151,9 +158,11 @@ int ptmf_mg_level_op(ptmf_mg mg, int level, ptmf_op *op);
 int ptmf_mg_set_coordinates(ptmf_mg mg, const double *coords);
 /* quadrature-point viscosity of one level (host, 27E, element-major) */
 int ptmf_mg_set_viscosity(ptmf_mg mg, int level, const double *eta);
-/* fill every coarser level from the finest: each coarse element takes the mean
- * of its 8 children's quadrature values */
-int ptmf_mg_coarsen_viscosity(ptmf_mg mg);
+/* fill every coarser level from the next finer one: each coarse element takes
+ * one mean of its 8 children's 216 quadrature values; mode 0 arithmetic,
+ * 1 geometric, 2 harmonic.  (The reference projects marker viscosity through a
+ * Q1 nodal field per level, material_point_utils.c:1267-1420.) */
+int ptmf_mg_coarsen_viscosity(ptmf_mg mg, int mode);
 /* Dirichlet mask of one level (host, 3N bytes) */
 int ptmf_mg_set_dirichlet(ptmf_mg mg, int level, const unsigned char *mask);
 /* diagonals, Jacobi scalings, Chebyshev eigenvalue bounds, coarse factor */
@@ -162,6 +171,9 @@ int ptmf_mg_level_eigs(ptmf_mg mg, int level, double *emin, double *emax);
 int ptmf_mg_set_level_eigs(ptmf_mg mg, int level, double emin, double emax);
 /* one V-cycle on the finest level: x = M^-1 b (device FP64 arrays) */
 int ptmf_mg_vcycle(ptmf_mg mg, const double *b, double *x);
+/* y = A_level x with the level's operator (rediscretised matrix-free or
+ * Galerkin stencil), precision 64 (double*) or 32 (float*) */
+int ptmf_mg_level_apply(ptmf_mg mg, int level, int precision, const void *x, void *y);
 /* yf += P xc and yc = P^T xf between levels level-1 and level (device FP64) */
 int ptmf_mg_prolong_add(ptmf_mg mg, int level, const double *xc, double *yf);
 int ptmf_mg_restrict(ptmf_mg mg, int level, const double *xf, double *yc);

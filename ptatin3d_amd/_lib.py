@@ -66,12 +66,13 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_mg_level_op": (_I, [_P, _I, C.POINTER(_P)]),
     "ptmf_mg_set_coordinates": (_I, [_P, _P]),
     "ptmf_mg_set_viscosity": (_I, [_P, _I, _P]),
-    "ptmf_mg_coarsen_viscosity": (_I, [_P]),
+    "ptmf_mg_coarsen_viscosity": (_I, [_P, _I]),
     "ptmf_mg_set_dirichlet": (_I, [_P, _I, _P]),
     "ptmf_mg_setup": (_I, [_P, _P]),
     "ptmf_mg_level_eigs": (_I, [_P, _I, _PD, _PD]),
     "ptmf_mg_set_level_eigs": (_I, [_P, _I, _D, _D]),
     "ptmf_mg_vcycle": (_I, [_P, _P, _P]),
+    "ptmf_mg_level_apply": (_I, [_P, _I, _I, _P, _P]),
     "ptmf_mg_prolong_add": (_I, [_P, _I, _P, _P]),
     "ptmf_mg_restrict": (_I, [_P, _I, _P, _P]),
     "ptmf_mg_smooth": (_I, [_P, _I, _I, _I, _I, _P, _P]),
@@ -92,6 +93,8 @@ class MGOptions(C.Structure):
         ("esteig_its", C.c_int),
         ("coarse_rtol", C.c_double),
         ("coarse_maxit", C.c_int),
+        ("galerkin_levels", C.c_int),
+        ("bc_transfer", C.c_int),
     ]
 
 

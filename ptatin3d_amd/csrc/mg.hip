@@ -254,9 +254,10 @@ __global__ void k_restrict(long Xc, long Yc, long Zc, long Xf, long Yf, long Zf,
   }
 }
 
-// coarse-element viscosity: mean of the 8 x 27 quadrature values of the
-// children (element-major input and output)
-__global__ void k_coarsen_eta(int mxc, int myc, int mzc, const double *__restrict__ ef,
+// coarse-element viscosity from the 8 x 27 quadrature values of the children
+// (element-major input and output): mode 0 arithmetic, 1 geometric,
+// 2 harmonic mean, one value per coarse element
+__global__ void k_coarsen_eta(int mxc, int myc, int mzc, int mode, const double *__restrict__ ef,
                               double *__restrict__ ec) {
   const long nec = (long)mxc * myc * mzc;
   const int mxf = 2 * mxc, myf = 2 * myc;
@@ -267,11 +268,102 @@ __global__ void k_coarsen_eta(int mxc, int myc, int mzc, const double *__restric
       for (int b = 0; b < 2; b++)
         for (int a = 0; a < 2; a++) {
           const long ef_e = (2 * ei + a) + (long)mxf * ((2 * ej + b) + (long)myf * (2 * ek + c));
-          for (int q = 0; q < 27; q++) s += ef[27 * ef_e + q];
+          for (int q = 0; q < 27; q++) {
+            const double v = ef[27 * ef_e + q];
+            s += mode == 0 ? v : (mode == 1 ? log(v) : 1.0 / v);
+          }
         }
     s *= 1.0 / 216.0;
-    for (int q = 0; q < 27; q++) ec[27 * E + q] = s;
+    const double m = mode == 0 ? s : (mode == 1 ? exp(s) : 1.0 / s);
+    for (int q = 0; q < 27; q++) ec[27 * E + q] = m;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Galerkin coarse operators stored as 125-point nodal stencils with 3x3
+// blocks, offset-major for coalesced reads: S[((o * 9) + 3 r + c) * N + m] is
+// the (r, c) block entry coupling node m to node m + off(o), off(o) in
+// [-2, 2]^3, o = (oi+2) + 5 (oj+2) + 25 (ok+2).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_stencil_apply(long X, long Y, long Z, const T *__restrict__ S, const T *__restrict__ x,
+                    T *__restrict__ y) {
+  const long N = X * Y * Z;
+  for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < N; m += (long)gridDim.x * blockDim.x) {
+    const long i = m % X, j = (m / X) % Y, k = m / (X * Y);
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int ok = -2; ok <= 2; ok++) {
+      const long kk = k + ok;
+      if (kk < 0 || kk >= Z) continue;
+      for (int oj = -2; oj <= 2; oj++) {
+        const long jj = j + oj;
+        if (jj < 0 || jj >= Y) continue;
+#pragma unroll
+        for (int oi = -2; oi <= 2; oi++) {
+          const long ii = i + oi;
+          if (ii < 0 || ii >= X) continue;
+          const int o = (oi + 2) + 5 * (oj + 2) + 25 * (ok + 2);
+          const long n = ii + X * (jj + Y * kk);
+          const T x0 = x[3 * n], x1 = x[3 * n + 1], x2 = x[3 * n + 2];
+          const T *b = S + (long)o * 9 * N + m;
+          a0 += b[0] * x0 + b[N] * x1 + b[2 * N] * x2;
+          a1 += b[3 * N] * x0 + b[4 * N] * x1 + b[5 * N] * x2;
+          a2 += b[6 * N] * x0 + b[7 * N] * x1 + b[8 * N] * x2;
+        }
+      }
+    }
+    y[3 * m] = a0;
+    y[3 * m + 1] = a1;
+    y[3 * m + 2] = a2;
+  }
+}
+
+// probe vector: v[3n + d] = 1 for the nodes of colour (ci, cj, ck) (indices
+// modulo 5), 0 elsewhere
+__global__ void k_probe(long X, long Y, long Z, int ci, int cj, int ck, int d, double *__restrict__ v) {
+  const long N = X * Y * Z;
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < N; n += (long)gridDim.x * blockDim.x) {
+    const long i = n % X, j = (n / X) % Y, k = n / (X * Y);
+    const bool on = (i % 5 == ci) && (j % 5 == cj) && (k % 5 == ck);
+    v[3 * n] = (on && d == 0) ? 1.0 : 0.0;
+    v[3 * n + 1] = (on && d == 1) ? 1.0 : 0.0;
+    v[3 * n + 2] = (on && d == 2) ? 1.0 : 0.0;
+  }
+}
+
+// row m of the probed product w = A v holds column (n, d) of A, n the unique
+// node of the colour within +-2 of m
+__global__ void k_probe_scatter(long X, long Y, long Z, int ci, int cj, int ck, int d,
+                                const double *__restrict__ w, double *__restrict__ S) {
+  const long N = X * Y * Z;
+  for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < N; m += (long)gridDim.x * blockDim.x) {
+    const long i = m % X, j = (m / X) % Y, k = m / (X * Y);
+    int oi = (int)(((ci - i) % 5 + 5) % 5), oj = (int)(((cj - j) % 5 + 5) % 5), ok = (int)(((ck - k) % 5 + 5) % 5);
+    if (oi > 2) oi -= 5;
+    if (oj > 2) oj -= 5;
+    if (ok > 2) ok -= 5;
+    if (i + oi < 0 || i + oi >= X || j + oj < 0 || j + oj >= Y || k + ok < 0 || k + ok >= Z) continue;
+    const int o = (oi + 2) + 5 * (oj + 2) + 25 * (ok + 2);
+    for (int r = 0; r < 3; r++) S[((long)o * 9 + 3 * r + d) * N + m] = w[3 * m + r];
+  }
+}
+
+// Dirichlet dof g = 3m + d of a Galerkin level: identity row (its row and
+// column entries are already zero from the masked probes)
+__global__ void k_stencil_bc_identity(long N, const long *__restrict__ idx, long nbc, double *__restrict__ S) {
+  const int oc = 62;  // offset (0,0,0)
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < nbc; k += (long)gridDim.x * blockDim.x) {
+    const long g = idx[k], m = g / 3;
+    const int d = (int)(g - 3 * m);
+    S[((long)oc * 9 + 4 * d) * N + m] = 1.0;
+  }
+}
+
+__global__ void k_stencil_diag(long N, const double *__restrict__ S, double *__restrict__ dg) {
+  const int oc = 62;  // offset (0,0,0)
+  for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < N; m += (long)gridDim.x * blockDim.x)
+    for (int r = 0; r < 3; r++) dg[3 * m + r] = S[((long)oc * 9 + 4 * r) * N + m];
 }
 
 }  // namespace
@@ -287,6 +379,9 @@ struct MGLevel {
   float *dinv32 = nullptr;
   double emin = 0, emax = 0;
   bool have_eta = false, have_bc = false;
+  bool galerkin = false;  // level operator = R A_{l+1} P, stored as a stencil
+  double *S64 = nullptr;
+  float *S32 = nullptr;
   // work vectors
   double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr, *p64[3] = {nullptr, nullptr, nullptr};
   float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *p32[3] = {nullptr, nullptr, nullptr};
@@ -373,7 +468,12 @@ int scale(hipStream_t s, long n, T a, const T *x, T *y) {
 
 template <typename T>
 int matmult(const MGLevel &L, const T *x, T *y) {
-  return launch_a11_mult<T>(L.op, x, y, true);
+  if (!L.galerkin) return launch_a11_mult<T>(L.op, x, y, true);
+  const Box &b = L.op->box;
+  const T *S = sizeof(T) == 8 ? (const T *)L.S64 : (const T *)L.S32;
+  k_stencil_apply<T><<<nblocks(b.nnodes), kBS, 0, L.op->stream>>>(b.X, b.Y, b.Z, S, x, y);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
 }
 
 // Chebyshev-Jacobi smoothing on level L (precision T): x <- cheb(b, x)
@@ -462,6 +562,10 @@ int vcycle(ptmf_mg_s *mg, int l, int lc, const T *b, T *x) {
   const Box &bf = L.op->box, &bcx = C.op->box;
   k_restrict<T><<<nblocks(bcx.nnodes), kBS, 0, s>>>(bcx.X, bcx.Y, bcx.Z, bf.X, bf.Y, bf.Z, r, bc);
   PTMF_HIP(hipGetLastError());
+  // coarse correction vanishing on the Dirichlet dofs: the coarse Dirichlet
+  // rows are identities, so a nonzero restricted residual there would act as
+  // boundary data of the correction
+  if (mg->opt.bc_transfer) PTMF_TRY(launch_bc_set<T>(C.op->bcidx, C.op->nbc, (T)0, bc, s));
   PTMF_TRY(vcycle<T>(mg, l - 1, lc, bc, xc));
   k_prolong_add<T><<<nblocks(bf.nnodes), kBS, 0, s>>>(bcx.X, bcx.Y, bf.X, bf.Y, bf.Z, xc, x);
   PTMF_HIP(hipGetLastError());
@@ -590,11 +694,48 @@ int estimate_emax(ptmf_mg_s *mg, int l, int its, double *emax) {
   return PTMF_OK;
 }
 
+// Galerkin operator of level l: S_l = R A_{l+1} P by probing with the 375
+// colourings (node indices modulo 5, one component each); the coupling range
+// of R A P stays within +-2 nodes, so every probe yields one stencil entry
+// per row and component.  P acts on the coarse non-Dirichlet dofs only (the
+// probes are masked and the product's Dirichlet rows dropped); Dirichlet dofs
+// get identity rows, as the rediscretised operators have.
+int assemble_galerkin(ptmf_mg_s *mg, int l) {
+  MGLevel &Cl = mg->lev[l], &Fl = mg->lev[l + 1];
+  hipStream_t s = mg->stream;
+  const Box &bc = Cl.op->box, &bf = Fl.op->box;
+  const long N = bc.nnodes;
+  if (!Cl.S64) PTMF_TRY(dalloc((void **)&Cl.S64, sizeof(double) * 1125 * N));
+  if (!Cl.S32) PTMF_TRY(dalloc((void **)&Cl.S32, sizeof(float) * 1125 * N));
+  PTMF_HIP(hipMemsetAsync(Cl.S64, 0, sizeof(double) * 1125 * N, s));
+  double *v = Cl.p64[0], *w = Cl.r64, *pf = Fl.p64[0], *apf = Fl.p64[1];
+  for (int ck = 0; ck < 5; ck++)
+    for (int cj = 0; cj < 5; cj++)
+      for (int ci = 0; ci < 5; ci++)
+        for (int d = 0; d < 3; d++) {
+          k_probe<<<nblocks(N), kBS, 0, s>>>(bc.X, bc.Y, bc.Z, ci, cj, ck, d, v);
+          PTMF_TRY(launch_bc_set<double>(Cl.op->bcidx, Cl.op->nbc, 0.0, v, s));
+          PTMF_HIP(hipMemsetAsync(pf, 0, sizeof(double) * Fl.n, s));
+          k_prolong_add<double><<<nblocks(bf.nnodes), kBS, 0, s>>>(bc.X, bc.Y, bf.X, bf.Y, bf.Z, v, pf);
+          PTMF_TRY(matmult<double>(Fl, pf, apf));
+          k_restrict<double><<<nblocks(N), kBS, 0, s>>>(bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z, apf, w);
+          PTMF_TRY(launch_bc_set<double>(Cl.op->bcidx, Cl.op->nbc, 0.0, w, s));
+          k_probe_scatter<<<nblocks(N), kBS, 0, s>>>(bc.X, bc.Y, bc.Z, ci, cj, ck, d, w, Cl.S64);
+          PTMF_HIP(hipGetLastError());
+        }
+  if (Cl.op->nbc > 0)
+    k_stencil_bc_identity<<<nblocks(Cl.op->nbc), kBS, 0, s>>>(N, Cl.op->bcidx, Cl.op->nbc, Cl.S64);
+  k_cast<double, float><<<nblocks(1125 * N), kBS, 0, s>>>(1125 * N, Cl.S64, Cl.S32);
+  PTMF_HIP(hipGetLastError());
+  Cl.galerkin = true;
+  return PTMF_OK;
+}
+
 int assemble_dense(ptmf_mg_s *mg) {
   MGLevel &L = mg->lev[0];
   const long n = L.n;
-  PTMF_CHECK(n <= 6000, PTMF_ERR_UNSUPPORTED,
-             "coarsest level has %ld dofs; the dense coarse solve takes at most 6000", n);
+  PTMF_CHECK(n <= 24000, PTMF_ERR_UNSUPPORTED,
+             "coarsest level has %ld dofs; the dense coarse solve takes at most 24000", n);
   mg->ndense = (int)n;
   if (!mg->Adense) PTMF_TRY(dalloc((void **)&mg->Adense, sizeof(double) * n * n));
   hipStream_t s = mg->stream;
@@ -642,6 +783,8 @@ int ptmf_mg_default_options(ptmf_mg_options *o) {
   o->esteig_its = 20;
   o->coarse_rtol = 1e-3;
   o->coarse_maxit = 200;
+  o->galerkin_levels = 0;
+  o->bc_transfer = 1;
   return PTMF_OK;
 }
 
@@ -714,7 +857,7 @@ int ptmf_mg_destroy(ptmf_mg *pmg) {
       ptmf_op_destroy(&L.op);
     }
     void *ps[] = {L.diag, L.dinv64, L.b64, L.x64, L.r64, L.p64[0], L.p64[1], L.p64[2],
-                  L.dinv32, L.b32, L.x32, L.r32, L.p32[0], L.p32[1], L.p32[2]};
+                  L.dinv32, L.b32, L.x32, L.r32, L.p32[0], L.p32[1], L.p32[2], L.S64, L.S32};
     for (void *p : ps) (void)hipFree(p);
   }
   for (double *p : mg->V) (void)hipFree(p);
@@ -780,8 +923,9 @@ int ptmf_mg_set_viscosity(ptmf_mg mg, int level, const double *eta) {
   return PTMF_OK;
 }
 
-int ptmf_mg_coarsen_viscosity(ptmf_mg mg) {
+int ptmf_mg_coarsen_viscosity(ptmf_mg mg, int mode) {
   PTMF_CHECK(mg, PTMF_ERR_ARG, "mg_coarsen_viscosity: null");
+  PTMF_CHECK(mode >= 0 && mode <= 2, PTMF_ERR_UNSUPPORTED, "mg_coarsen_viscosity: mode %d (0 arithmetic, 1 geometric, 2 harmonic)", mode);
   PTMF_CHECK(mg->lev[mg->L - 1].have_eta, PTMF_ERR_NOT_SETUP, "mg_coarsen_viscosity: finest viscosity not set");
   for (int l = mg->L - 1; l > 0; l--) {
     MGLevel &F = mg->lev[l], &Cl = mg->lev[l - 1];
@@ -795,7 +939,7 @@ int ptmf_mg_coarsen_viscosity(ptmf_mg mg) {
     for (long e = 0; e < nef; e++)
       for (int q = 0; q < 27; q++) he[27 * e + q] = hq[(long)q * nef + e];
     PTMF_HIP(hipMemcpy(ef, he.data(), sizeof(double) * 27 * nef, hipMemcpyHostToDevice));
-    ptmf::k_coarsen_eta<<<ptmf::nblocks(nec), ptmf::kBS, 0, mg->stream>>>(Cl.mx, Cl.my, Cl.mz, ef, ec);
+    ptmf::k_coarsen_eta<<<ptmf::nblocks(nec), ptmf::kBS, 0, mg->stream>>>(Cl.mx, Cl.my, Cl.mz, mode, ef, ec);
     PTMF_HIP(hipGetLastError());
     std::vector<double> hc(27 * nec);
     PTMF_HIP(hipMemcpyAsync(hc.data(), ec, sizeof(double) * 27 * nec, hipMemcpyDeviceToHost, mg->stream));
@@ -825,12 +969,27 @@ int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt) {
   PTMF_CHECK(o.smoother_precision == 32 || o.smoother_precision == 64, PTMF_ERR_UNSUPPORTED,
              "mg_setup: smoother precision %d (32 or 64)", o.smoother_precision);
   PTMF_CHECK(o.pre_its >= 0 && o.post_its >= 0 && o.esteig_its > 0, PTMF_ERR_ARG, "mg_setup: bad iteration counts");
+  PTMF_CHECK(o.galerkin_levels >= 0 && o.galerkin_levels < mg->L, PTMF_ERR_ARG,
+             "mg_setup: galerkin_levels %d (0..nlevels-1)", o.galerkin_levels);
   hipStream_t s = mg->stream;
+  // Galerkin coarse operators, built top-down from the finer level's operator
+  for (int l = mg->L - 1; l >= 0; l--) mg->lev[l].galerkin = false;
+  for (int l = mg->L - 2; l >= 0; l--) {
+    MGLevel &L = mg->lev[l];
+    PTMF_CHECK(L.op->has_coords && L.have_bc, PTMF_ERR_NOT_SETUP,
+               "mg_setup: level %d lacks coordinates or Dirichlet list", l);
+    if (l < o.galerkin_levels) PTMF_TRY(ptmf::assemble_galerkin(mg, l));
+  }
   for (int l = 0; l < mg->L; l++) {
     MGLevel &L = mg->lev[l];
-    PTMF_CHECK(L.op->has_coords && L.have_eta && L.have_bc, PTMF_ERR_NOT_SETUP,
+    PTMF_CHECK(L.op->has_coords && (L.have_eta || L.galerkin) && L.have_bc, PTMF_ERR_NOT_SETUP,
                "mg_setup: level %d lacks coordinates, viscosity or Dirichlet list", l);
-    PTMF_TRY(ptmf_a11_diagonal(L.op, L.diag));
+    if (L.galerkin) {
+      ptmf::k_stencil_diag<<<ptmf::nblocks(L.op->box.nnodes), ptmf::kBS, 0, s>>>(L.op->box.nnodes, L.S64, L.diag);
+      PTMF_HIP(hipGetLastError());
+    } else {
+      PTMF_TRY(ptmf_a11_diagonal(L.op, L.diag));
+    }
     ptmf::k_recip<double><<<ptmf::nblocks(L.n), ptmf::kBS, 0, s>>>(L.n, L.diag, L.dinv64);
     ptmf::k_recip<float><<<ptmf::nblocks(L.n), ptmf::kBS, 0, s>>>(L.n, L.diag, L.dinv32);
     PTMF_HIP(hipGetLastError());
@@ -886,6 +1045,19 @@ int ptmf_mg_vcycle(ptmf_mg mg, const double *b, double *x) {
     PTMF_TRY(ptmf::vcycle<float>(mg, lf, lc, F.b32, F.x32));
     PTMF_TRY((ptmf::cast<float, double>(mg->stream, F.n, F.x32, x)));
   }
+  return PTMF_OK;
+}
+
+int ptmf_mg_level_apply(ptmf_mg mg, int level, int precision, const void *x, void *y) {
+  PTMF_CHECK(mg && x && y && level >= 0 && level < mg->L, PTMF_ERR_ARG, "mg_level_apply: bad level %d", level);
+  PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_level_apply: not set up");
+  if (precision == 64)
+    PTMF_TRY(ptmf::matmult<double>(mg->lev[level], (const double *)x, (double *)y));
+  else if (precision == 32)
+    PTMF_TRY(ptmf::matmult<float>(mg->lev[level], (const float *)x, (float *)y));
+  else
+    return ptmf_set_error(PTMF_ERR_UNSUPPORTED, "mg_level_apply: precision %d", precision);
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
   return PTMF_OK;
 }
 

@@ -110,6 +110,21 @@ def eta_layered_ellipsoids(qpx: np.ndarray, n_ell=64, rng=None) -> np.ndarray:
     return eta
 
 
+def smooth_body_force(coords, mx, my, mz, rng=None) -> np.ndarray:
+    """Smooth body-force load: f = a * (sin(pi x) sin(pi y), sin(pi y) sin(pi z),
+    sin(pi z) sin(pi x)) with seeded amplitudes a_d in [0.5, 1.5], lumped to
+    the nodes with the nodal volume 1 / (8 mx my mz); Dirichlet rows (all
+    faces) zero."""
+    rng = rng or np.random.default_rng(SEED + 5)
+    a = rng.uniform(0.5, 1.5, size=3)
+    c = coords.reshape(-1, 3)
+    s = np.sin(np.pi * c)
+    f = np.stack([a[0] * s[:, 0] * s[:, 1], a[1] * s[:, 1] * s[:, 2], a[2] * s[:, 2] * s[:, 0]], axis=1)
+    f = f.reshape(-1) / (8.0 * mx * my * mz)
+    f[dirichlet_all_faces(mx, my, mz).astype(bool)] = 0.0
+    return f
+
+
 def random_velocity(mx, my, mz, bcmask=None, rng=None) -> np.ndarray:
     rng = rng or np.random.default_rng(SEED + 4)
     u = rng.uniform(-1.0, 1.0, size=3 * nnodes(mx, my, mz))
@@ -145,3 +160,14 @@ def config2(m=64, seed=SEED):
     bc = dirichlet_all_faces(m, m, m)
     u = random_velocity(m, m, m, bc, rng)
     return dict(mx=m, my=m, mz=m, coords=coords, eta=eta, bc=bc, u=u)
+
+
+def config3(m=128, seed=SEED):
+    """128^3 unit cube, layered viscosity 1e3/1e1/1e-3 with 64 weak ellipsoids
+    (contrast 1e6), smooth body-force load."""
+    rng = np.random.default_rng(seed)
+    coords = box_coords(m, m, m)
+    eta = eta_layered_ellipsoids(qp_coordinates(coords, m, m, m), rng=rng)
+    bc = dirichlet_all_faces(m, m, m)
+    b = smooth_body_force(coords, m, m, m, rng)
+    return dict(mx=m, my=m, mz=m, coords=coords, eta=eta, bc=bc, b=b)

@@ -62,8 +62,9 @@ class GMG:
         e = np.ascontiguousarray(eta, dtype=np.float64)
         call("ptmf_mg_set_viscosity", self.handle, int(level), _lib._hp(e))
 
-    def coarsen_viscosity(self) -> None:
-        call("ptmf_mg_coarsen_viscosity", self.handle)
+    def coarsen_viscosity(self, mode: str = "arithmetic") -> None:
+        modes = {"arithmetic": 0, "geometric": 1, "harmonic": 2}
+        call("ptmf_mg_coarsen_viscosity", self.handle, modes[mode])
 
     def set_dirichlet(self, level: int, mask: np.ndarray) -> None:
         m = np.ascontiguousarray(mask, dtype=np.uint8)
@@ -88,6 +89,10 @@ class GMG:
         h = C.c_void_p()
         call("ptmf_mg_level_op", self.handle, int(level), C.byref(h))
         return h
+
+    def level_apply(self, level: int, x: DeviceArray, y: DeviceArray) -> None:
+        prec = 32 if x.dtype == np.float32 else 64
+        call("ptmf_mg_level_apply", self.handle, int(level), prec, x.ptr, y.ptr)
 
     def vcycle(self, b: DeviceArray, x: DeviceArray) -> None:
         call("ptmf_mg_vcycle", self.handle, b.ptr, x.ptr)
