@@ -6,7 +6,15 @@ evaluated at the 27 Gauss points per element), random input velocity with
 Dirichlet rows on all faces.  One step = one MatMult with vectors resident in
 HBM (column masking, element apply, Dirichlet rows).  Prints one JSON line.
 
+The MatMult tiling and kernel variant are picked by the autotuner before the
+timed region.  --solve adds the mixed-precision multigrid solve of the
+layered 1e6-contrast workload (FP32 V-cycle vs FP64 V-cycle, FGMRES in FP64,
+both to a 1e-8 relative FP64 residual) as a "solve" object.  --copy-probe
+runs one plain device copy of 256 MiB after the timed region (the reference
+kernel for HBM byte counters).
+
   python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--m 64] [--no-cpu]
+                  [--solve [--solve-m 128]] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -46,6 +54,10 @@ def main() -> int:
     ap.add_argument("--m", type=int, default=64, help="elements per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tiling")
+    ap.add_argument("--solve", action="store_true", help="add the mixed-precision GMG solve leg")
+    ap.add_argument("--solve-m", type=int, default=128)
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -53,8 +65,8 @@ def main() -> int:
                           "n_gpus": args.gpus}), flush=True)
         return 2
 
-    from ptatin3d_amd import fields as F
-    from ptatin3d_amd._lib import DeviceArray, device_count
+    from ptatin3d_amd import autotune, fields as F
+    from ptatin3d_amd._lib import DeviceArray, call, device_count
     from ptatin3d_amd.operators import MatA11MF
 
     if device_count() < 1:
@@ -69,11 +81,14 @@ def main() -> int:
     A = MatA11MF(m, m, m).setup(c["coords"], c["eta"], c["bc"])
     x = DeviceArray.from_host(c["u"])
     y = DeviceArray(ndof)
+    if not args.no_tune:
+        autotune.tune(A, precision=64, reps=10, max_shapes=4)
+    tiling = A.get_tiling()
+    variant = A.get_kernel_variant()
 
     # warmup, then the timed region: K MatMults bracketed by device syncs
     if args.warmup > 0:
         A.time_mult(x, y, args.warmup)
-    from ptatin3d_amd._lib import call
     call("ptmf_device_synchronize")
     t0 = time.perf_counter()
     ms_event = A.time_mult(x, y, args.steps)
@@ -116,6 +131,8 @@ def main() -> int:
             "elements": E,
             "velocity_dofs": ndof,
             "parallelism": "single GPU",
+            "tiling": list(tiling),
+            "kernel_variant": variant,
         },
         "roofline": {
             "bound": "mfma",
@@ -125,7 +142,8 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": round(tflops / PEAK_FP64_TFLOPS, 4),
             "traffic": None,
-            "kernel": "a11_apply_kernel<double>",
+            "kernel": ("a11_tile_pf_kernel<double, true>" if variant == 1
+                       else "a11_tile_kernel<double, true>"),
             "kernel_ms": round(ms_kernel, 5),
             "alg_flops_per_launch": flops,
             "alg_bytes_per_launch": alg_bytes,
@@ -161,9 +179,57 @@ def main() -> int:
         err = float(np.abs(y_gpu - y_cpu).max() / np.abs(y_cpu).max())
         out["parity_rel_inf_vs_oracle"] = err
 
-    print(json.dumps(out), flush=True)
+    if args.copy_probe:
+        nb = 256 << 20
+        src, dst = DeviceArray(nb // 8), DeviceArray(nb // 8)
+        call("ptmf_memcpy_dtod", dst.ptr, src.ptr, nb)
+        out["copy_probe_bytes"] = nb
+        src.free()
+        dst.free()
+
     A.destroy()
+    if args.solve:
+        out["solve"] = solve_leg(args.solve_m)
+    print(json.dumps(out), flush=True)
     return 0
+
+
+def solve_leg(m: int) -> dict:
+    """Layered viscosity 1e3/1e1/1e-3 with 64 weak ellipsoids (contrast 1e6),
+    smooth body force; FGMRES (FP64) preconditioned by a geometric V-cycle
+    with Chebyshev(3)-Jacobi smoothing in FP32 or FP64 and Galerkin coarse
+    levels; both solved to ||b - A x|| <= 1e-8 ||b|| in FP64.  The error of
+    each against a 1e-11 FP64 solve is reported."""
+    from ptatin3d_amd import fields as F
+    from ptatin3d_amd.multigrid import GMG, default_options, solve_timed
+
+    c = F.config3(m)
+    nl = 1
+    while (m >> nl) >= 4 and nl < 6:
+        nl += 1
+    res = {"workload": f"{m}^3 Q2 elements, layered 1e3/1e1/1e-3 + 64 weak ellipsoids, "
+                       "smooth body force, all-face Dirichlet",
+           "velocity_dofs": 3 * F.nnodes(m, m, m), "levels": nl}
+    runs = {}
+    for tag, prec, rtol in (("fp64", 64, 1e-8), ("fp32", 32, 1e-8), ("tight", 64, 1e-11)):
+        mg = GMG(m, m, m, nl)
+        mg.set_coordinates(c["coords"])
+        mg.set_viscosity(nl - 1, c["eta"])
+        mg.coarsen_viscosity("arithmetic")
+        mg.set_dirichlet_all_faces()
+        t0 = time.perf_counter()
+        mg.setup(default_options(smoother_precision=prec, galerkin_levels=nl - 1))
+        ts = time.perf_counter() - t0
+        x, its, rn, hist, secs = solve_timed(mg, c["b"], rtol=rtol, maxit=1000, restart=100)
+        runs[tag] = x
+        res[tag] = {"its": its, "seconds": round(secs, 4), "setup_seconds": round(ts, 3),
+                    "rel_residual": float(rn / np.linalg.norm(c["b"]))}
+        mg.destroy()
+    ref = runs["tight"]
+    for tag in ("fp64", "fp32"):
+        res[tag]["rel_err_vs_tight"] = float(np.linalg.norm(runs[tag] - ref) / np.linalg.norm(ref))
+    res["speedup_fp32_vs_fp64"] = round(res["fp64"]["seconds"] / res["fp32"]["seconds"], 3)
+    return res
 
 
 if __name__ == "__main__":

@@ -86,6 +86,8 @@ int ptmf_malloc(void **dptr, size_t bytes);
 int ptmf_free(void *dptr);
 int ptmf_memcpy_htod(void *dst, const void *src, size_t bytes);
 int ptmf_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+/* device-to-device copy with a plain streaming kernel (16-byte aligned) */
+int ptmf_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int ptmf_memset(void *dptr, int value, size_t bytes);
 int ptmf_device_synchronize(void);
 
@@ -109,6 +111,13 @@ int ptmf_op_set_viscosity_device(ptmf_op op, const double *eta);
  * coloring, stokes_operators_cuda.cu:337-397, plays this role there). */
 int ptmf_op_set_tiling(ptmf_op op, int TX, int TY, int ZC);
 int ptmf_op_get_tiling(ptmf_op op, int *TX, int *TY, int *ZC);
+/* MatMult kernel variant: 0 = z-march; 1 = z-march with the next layer's
+ * loads issued one stage ahead (software pipelined; bit-identical to 0);
+ * 2 = stored quadrature-point geometry (partial assembly: 9 values per
+ * quadrature point, rebuilt after a coefficient or tiling change; same
+ * operator to rounding, needs positive 2 w eta / det) */
+int ptmf_op_set_kernel_variant(ptmf_op op, int variant);
+int ptmf_op_get_kernel_variant(ptmf_op op, int *variant);
 /* coefficient state counter, bumped by every set_* call (replaces the
  * PetscObjectState stamp of stokes_operators.c:1162-1163) */
 int ptmf_op_state(ptmf_op op, unsigned long *state);
@@ -186,6 +195,11 @@ int ptmf_mg_smooth(ptmf_mg mg, int level, int precision, int its, int nonzero_gu
  * FGMRES estimate after each iteration. */
 int ptmf_mg_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int restart,
                    int *its, double *rnorm, double *history, int history_len);
+/* Flexible CG preconditioned by the V-cycle, from the given x (a PCG-type
+ * outer iteration for the symmetric A11 block; no restart, 5 work vectors).
+ * Same stopping rule and outputs as ptmf_mg_fgmres. */
+int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int *its,
+                double *rnorm, double *history, int history_len);
 
 #ifdef __cplusplus
 }

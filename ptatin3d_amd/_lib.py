@@ -38,6 +38,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_free": (_I, [_P]),
     "ptmf_memcpy_htod": (_I, [_P, _P, C.c_size_t]),
     "ptmf_memcpy_dtoh": (_I, [_P, _P, C.c_size_t]),
+    "ptmf_memcpy_dtod": (_I, [_P, _P, C.c_size_t]),
     "ptmf_memset": (_I, [_P, _I, C.c_size_t]),
     "ptmf_device_synchronize": (_I, []),
     "ptmf_op_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
@@ -50,6 +51,8 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_op_set_viscosity_device": (_I, [_P, _P]),
     "ptmf_op_state": (_I, [_P, C.POINTER(C.c_ulong)]),
     "ptmf_op_set_tiling": (_I, [_P, _I, _I, _I]),
+    "ptmf_op_set_kernel_variant": (_I, [_P, _I]),
+    "ptmf_op_get_kernel_variant": (_I, [_P, _P]),
     "ptmf_op_get_tiling": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "ptmf_a11_spmv": (_I, [_P, _P, _P]),
     "ptmf_a11_spmv_host": (_I, [_P, _P, _P]),
@@ -77,6 +80,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_mg_restrict": (_I, [_P, _I, _P, _P]),
     "ptmf_mg_smooth": (_I, [_P, _I, _I, _I, _I, _P, _P]),
     "ptmf_mg_fgmres": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _PD, _PD, _I]),
+    "ptmf_mg_fcg": (_I, [_P, _P, _P, _D, _I, C.POINTER(_I), _PD, _PD, _I]),
     "ptmf_a11_time_kernel": (_I, [_P, _I, _P, _P, _I, _PD]),
 }
 

@@ -4,9 +4,10 @@ The MatMult kernel assigns a TX x TY tile of elements (TX*TY <= 21, one
 64-lane wave, 3 lanes per element) to a wave that marches through ZC element
 layers.  The best shape depends on the box (partial tiles waste lanes, thin
 tiles raise face-buffer traffic, long z-chunks leave too few waves to fill
-256 CUs).  `tune(op, ...)` times candidate shapes with HIP events on the
-operator's own stream and installs the fastest one; results are cached per
-(mesh, precision) in memory.
+256 CUs), and so does the choice between the plain and the software-pipelined
+kernel.  `tune(op, ...)` times candidate (shape, variant) pairs with HIP
+events on the operator's own stream and installs the fastest one; results are
+cached per (mesh, precision) in memory.
 """
 from __future__ import annotations
 
@@ -16,7 +17,8 @@ import numpy as np
 
 from ._lib import DeviceArray
 
-_CACHE: dict[tuple, tuple[int, int, int]] = {}
+_CACHE: dict[tuple, tuple[tuple[int, int, int], int]] = {}
+VARIANTS = (0, 1)
 
 
 def candidates(mx: int, my: int, mz: int, max_shapes: int = 8) -> list[tuple[int, int, int]]:
@@ -37,27 +39,34 @@ def candidates(mx: int, my: int, mz: int, max_shapes: int = 8) -> list[tuple[int
     return out
 
 
-def tune(op, precision: int = 64, reps: int = 10, max_shapes: int = 8, verbose: bool = False):
-    """Time the candidate decompositions on `op` and keep the fastest."""
+def tune(op, precision: int = 64, reps: int = 10, max_shapes: int = 8, verbose: bool = False,
+         variants=VARIANTS):
+    """Time the candidate (tiling, kernel variant) pairs on `op` and keep the
+    fastest.  Returns ((TX, TY, ZC), variant) and the sorted timings."""
     key = (op.mx, op.my, op.mz, precision)
     if key in _CACHE:
-        op.set_tiling(*_CACHE[key])
+        t, v = _CACHE[key]
+        op.set_tiling(*t)
+        op.set_kernel_variant(v)
         return _CACHE[key], []
     dt = np.float64 if precision == 64 else np.float32
     n = 3 * op.nnodes
     x = DeviceArray.from_host(np.random.default_rng(0).uniform(-1, 1, n).astype(dt))
     y = DeviceArray(n, dt)
     results = []
-    for t in candidates(op.mx, op.my, op.mz, max_shapes):
-        op.set_tiling(*t)
-        op.time_mult(x, y, 2)
-        ms = op.time_mult(x, y, reps)
-        results.append((ms, t))
-        if verbose:
-            print(f"  tiling {t}: {ms * 1e3:.1f} us", flush=True)
+    for v in variants:
+        op.set_kernel_variant(v)
+        for t in candidates(op.mx, op.my, op.mz, max_shapes):
+            op.set_tiling(*t)
+            op.time_mult(x, y, 2)
+            ms = op.time_mult(x, y, reps)
+            results.append((ms, (t, v)))
+            if verbose:
+                print(f"  tiling {t} variant {v}: {ms * 1e3:.1f} us", flush=True)
     results.sort()
     best = results[0][1]
-    op.set_tiling(*best)
+    op.set_tiling(*best[0])
+    op.set_kernel_variant(best[1])
     _CACHE[key] = best
     x.free()
     y.free()

@@ -48,26 +48,46 @@ __device__ __forceinline__ T sel3(int c, double a0, double a1, double a2) {
   return (T)(c == 0 ? a0 : (c == 1 ? a1 : a2));
 }
 
-// Load node plane (base + i + X*j, i,j = 0..2) of a 3-component field and
-// interpolate it to the 3x3 in-plane quadrature grid: V[d][3b+a] =
-// sum_{i,j} B[a][i] B[b][j] f_d(i,j).
+// Raw input of one lane's node plane: coordinates and velocity of the 9
+// nodes (i, j) of plane k = cp, their Dirichlet bits, and the viscosity at
+// the lane's 9 quadrature points.  Kept unmodified in registers between the
+// load and the forward stage so the loads can be issued a layer ahead.
+template <typename T>
+struct PlaneRaw {
+  T x[3][9];  // [d][3j+i]
+  T u[3][9];
+  unsigned m[9];
+  T eta[9];
+};
+
 template <typename T, bool kMask>
-__device__ __forceinline__ void load_interp_plane(const T *__restrict__ f, long base, long X,
-                                                  const uint8_t *__restrict__ bcn, T V[3][9]) {
-  T g[3][3][3];  // [d][j][i]
+__device__ __forceinline__ void load_plane_nodes(PlaneRaw<T> &r, const T *__restrict__ xc,
+                                                 const T *__restrict__ u,
+                                                 const uint8_t *__restrict__ bcn, long base, long X) {
 #pragma unroll
   for (int j = 0; j < 3; j++)
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       const long node = base + i + X * j;
-      unsigned m = 0;
-      if (kMask) m = bcn[node];
+      r.m[3 * j + i] = kMask ? (unsigned)bcn[node] : 0u;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        T v = f[3 * node + d];
-        g[d][j][i] = ((m >> d) & 1u) ? T(0) : v;
+        r.x[d][3 * j + i] = xc[3 * node + d];
+        r.u[d][3 * j + i] = u[3 * node + d];
       }
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_plane_eta(PlaneRaw<T> &r, const T *__restrict__ etap, long nel) {
+#pragma unroll
+  for (int p = 0; p < 9; p++) r.eta[p] = etap[(long)p * nel];
+}
+
+// in-plane interpolation of a 3x3 node plane to the 3x3 quadrature grid:
+// V[d][3b+a] = sum_{i,j} B[a][i] B[b][j] g[d][3j+i]
+template <typename T>
+__device__ __forceinline__ void interp_plane(const T g[3][9], T V[3][9]) {
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     T t[3][3];  // [j][a]
@@ -75,7 +95,7 @@ __device__ __forceinline__ void load_interp_plane(const T *__restrict__ f, long 
     for (int j = 0; j < 3; j++)
 #pragma unroll
       for (int a = 0; a < 3; a++)
-        t[j][a] = (T)kB[a][0] * g[d][j][0] + (T)kB[a][1] * g[d][j][1] + (T)kB[a][2] * g[d][j][2];
+        t[j][a] = (T)kB[a][0] * g[d][3 * j] + (T)kB[a][1] * g[d][3 * j + 1] + (T)kB[a][2] * g[d][3 * j + 2];
 #pragma unroll
     for (int b = 0; b < 3; b++)
 #pragma unroll
@@ -84,41 +104,18 @@ __device__ __forceinline__ void load_interp_plane(const T *__restrict__ f, long 
   }
 }
 
-// The element computation of one lane: element e (node plane origin `base`,
-// lane plane cp), result out[l][3j+i] = (A11_e u)(node (i,j,k=cp), comp l).
-// Every lane of the wave must call it (it contains wave-wide barriers); lanes
-// with writer == false (lane 63) never write the exchange buffer.
-//
-// Register budget: the in-plane values Xq, Uq and the accumulator W stay in
-// registers; the zeta derivatives are formed per quadrature point from the
-// exchange buffer, whose slots are then reused for the backward zeta data.
-// A work-group is a single wave, so the LDS operations of all lanes execute in
-// program order: a slot read by every lane at qp p is overwritten only after
-// those reads.
+// Forward stage: in-plane interpolation of the lane's node plane, exchange
+// through the LDS buffer (slots 0..26 coordinates, 27..53 masked velocity),
+// zeta interpolation to the lane's quadrature plane.
 template <typename T, bool kMask>
-__device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer, long e, long nel,
-                                                    long base, long X, const T *__restrict__ xc,
-                                                    const T *__restrict__ eta,
-                                                    const uint8_t *__restrict__ bcn,
-                                                    const T *__restrict__ u,
-                                                    T (*sh)[kSlots][kEPW], T out[3][9]) {
-  const T *etap = eta + (long)(9 * cp) * nel + e;
-  // zeta tables of this lane's plane: forward B[c][k], D[c][k]; backward
-  // (transposed) B[c'][k=c], D[c'][k=c].
-  T bz[3], dz[3];
+__device__ __forceinline__ void plane_forward(int cp, int le, bool writer, const PlaneRaw<T> &r,
+                                              T (*sh)[kSlots][kEPW], T Xq[3][9], T Uq[3][9]) {
+  T bz[3];
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    bz[k] = sel3<T>(cp, kB[0][k], kB[1][k], kB[2][k]);
-    dz[k] = sel3<T>(cp, kD[0][k], kD[1][k], kD[2][k]);
-  }
-  const T wz = sel3<T>(cp, kW[0], kW[1], kW[2]);
-
-  // ---- forward: in-plane interpolation of the node planes, exchanged so
-  // every lane sees all three planes (slots 0..26 coordinates, 27..53 velocity)
-  T Xq[3][9], Uq[3][9];
+  for (int k = 0; k < 3; k++) bz[k] = sel3<T>(cp, kB[0][k], kB[1][k], kB[2][k]);
   {
     T V[3][9];
-    load_interp_plane<T, false>(xc, base, X, nullptr, V);
+    interp_plane<T>(r.x, V);
     if (writer) {
 #pragma unroll
       for (int d = 0; d < 3; d++)
@@ -127,8 +124,12 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
     }
   }
   {
-    T V[3][9];
-    load_interp_plane<T, kMask>(u, base, X, bcn, V);
+    T g[3][9], V[3][9];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int n = 0; n < 9; n++) g[d][n] = ((r.m[n] >> d) & 1u) ? T(0) : r.u[d][n];
+    interp_plane<T>(g, V);
     if (writer) {
 #pragma unroll
       for (int d = 0; d < 3; d++)
@@ -146,8 +147,26 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
       Uq[d][p] = bz[0] * sh[0][27 + 9 * d + p][le] + bz[1] * sh[1][27 + 9 * d + p][le] +
                  bz[2] * sh[2][27 + 9 * d + p][le];
     }
+}
 
-  // ---- pointwise action on the 9 qps of this plane
+// Pointwise stage on the 9 quadrature points of the lane's plane: Jacobian
+// (adjugate form, one division per point), symmetric gradient, viscous
+// stress, and the transposed derivatives.  The xi/eta parts are accumulated
+// in W and exchanged with the zeta parts through the LDS buffer: on return,
+// slots 9l+p hold the zeta components and slots 27+9l+p the xi/eta
+// accumulations of every plane.
+//
+// A work-group is a single wave, so the LDS operations of all lanes execute
+// in program order: a slot read by every lane at qp p is overwritten only
+// after those reads.
+template <typename T>
+__device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq[3][9],
+                                         const T Uq[3][9], const T etq9[9],
+                                         T (*sh)[kSlots][kEPW]) {
+  T dz[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) dz[k] = sel3<T>(cp, kD[0][k], kD[1][k], kD[2][k]);
+  const T wz = sel3<T>(cp, kW[0], kW[1], kW[2]);
   T W[3][9];  // in-plane transposed xi/eta derivatives, accumulated
 #pragma unroll
   for (int l = 0; l < 3; l++)
@@ -159,7 +178,7 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       const int p = 3 * b + a;
-      const T etq = etap[(long)p * nel];
+      const T etq = etq9[p];
       // M[j][d] = d x_d / d xi_j, Gu[j][l] = d u_l / d xi_j
       T M[3][3], Gu[3][3];
 #pragma unroll
@@ -231,8 +250,14 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
       for (int p = 0; p < 9; p++) sh[cp][27 + 9 * l + p][le] = W[l][p];
   }
   __syncthreads();
+}
 
-  // ---- backward zeta stage: node plane k = c gathers all three qp planes
+// Backward stage: node plane k = cp gathers the three quadrature planes
+// (zeta), then the transposed in-plane interpolation; out[l][3j+i] is the
+// element result at node (i, j, k = cp), component l.  Ends with a barrier:
+// the exchange buffer is free again on return.
+template <typename T>
+__device__ __forceinline__ void plane_backward(int cp, int le, T (*sh)[kSlots][kEPW], T out[3][9]) {
   T bt[3], dt[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -249,10 +274,7 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
               w2 = sh[2][27 + 9 * l + p][le];
       Y[l][p] = bt[0] * w0 + bt[1] * w1 + bt[2] * w2 + dt[0] * z0 + dt[1] * z1 + dt[2] * z2;
     }
-  // the exchange buffer is reused by the caller's next element
   __syncthreads();
-
-  // ---- backward in-plane: transposed interpolation
 #pragma unroll
   for (int l = 0; l < 3; l++) {
     T t[3][3];  // [b][i]
@@ -268,6 +290,26 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
       for (int i = 0; i < 3; i++)
         out[l][3 * j + i] = (T)kB[0][j] * t[0][i] + (T)kB[1][j] * t[1][i] + (T)kB[2][j] * t[2][i];
   }
+}
+
+// The element computation of one lane: element e (node plane origin `base`,
+// lane plane cp), result out[l][3j+i] = (A11_e u)(node (i,j,k=cp), comp l).
+// Every lane of the wave must call it (it contains wave-wide barriers); lanes
+// with writer == false (lane 63) never write the exchange buffer.
+template <typename T, bool kMask>
+__device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer, long e, long nel,
+                                                    long base, long X, const T *__restrict__ xc,
+                                                    const T *__restrict__ eta,
+                                                    const uint8_t *__restrict__ bcn,
+                                                    const T *__restrict__ u,
+                                                    T (*sh)[kSlots][kEPW], T out[3][9]) {
+  PlaneRaw<T> r;
+  load_plane_nodes<T, kMask>(r, xc, u, bcn, base, X);
+  load_plane_eta<T>(r, eta + (long)(9 * cp) * nel + e, nel);
+  T Xq[3][9], Uq[3][9];
+  plane_forward<T, kMask>(cp, le, writer, r, sh, Xq, Uq);
+  plane_qp<T>(cp, le, writer, Xq, Uq, r.eta, sh);
+  plane_backward<T>(cp, le, sh, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -424,6 +466,355 @@ __global__ void __launch_bounds__(64)
   flush_plane<T>(a, tile, tb, 2 * (tb.z0 + tb.nzt), win[(2 * tb.nzt) % 3], y, fbx, fby, fbz);
 }
 
+// The same z-march, software-pipelined across layers: the raw node plane of
+// layer k+1 is loaded while layer k's backward stage runs, its viscosity
+// while layer k's window sums run, and layer k's completed node planes are
+// flushed after layer k+1's forward stage, so no global load is waited on
+// right after it is issued.
+template <typename T, bool kMask>
+__global__ void __launch_bounds__(64)
+    a11_tile_pf_kernel(TileArgs a, const T *__restrict__ xc, const T *__restrict__ eta,
+                       const uint8_t *__restrict__ bcn, const T *__restrict__ u, T *__restrict__ y,
+                       T *__restrict__ fbx, T *__restrict__ fby, T *__restrict__ fbz) {
+  __shared__ T sh[3][kSlots][kEPW];
+  __shared__ T win[3][3 * kWinNodes];
+  const long tile = blockIdx.x;
+  const TileBox tb = tile_box(a, tile);
+  const int lane = threadIdx.x;
+  const int c = lane / kEPW;
+  const int le = lane - c * kEPW;
+  const int cp = c < 3 ? c : 2;
+  const bool writer = c < 3;
+  const int lx = le % a.TX, ly = le / a.TX;
+  const bool live = writer && (le < a.TX * a.TY) && (lx < tb.nxt) && (ly < tb.nyt);
+  const int ex = tb.x0 + min(lx, tb.nxt - 1), ey = tb.y0 + min(ly, tb.nyt - 1);
+  const int wX = 2 * tb.nxt + 1;
+  const long ecol = ex + (long)a.mx * ey;       // element (ex, ey, 0)
+  const long ncol = 2L * ex + a.X * (2L * ey);  // its node (0, 0, 0)
+  const long nlay = (long)a.mx * a.my;
+
+  PlaneRaw<T> r;
+  load_plane_nodes<T, kMask>(r, xc, u, bcn, ncol + a.XY * (2L * tb.z0 + cp), a.X);
+  load_plane_eta<T>(r, eta + (long)(9 * cp) * a.nel + ecol + nlay * tb.z0, a.nel);
+  for (int t = lane; t < 9 * kWinNodes; t += 64) (&win[0][0])[t] = T(0);
+  __syncthreads();
+
+  for (int kz = 0; kz < tb.nzt; kz++) {
+    const long ek = tb.z0 + kz;
+    T out[3][9];
+    {
+      T Xq[3][9], Uq[3][9];
+      plane_forward<T, kMask>(cp, le, writer, r, sh, Xq, Uq);
+      plane_qp<T>(cp, le, writer, Xq, Uq, r.eta, sh);
+    }
+    plane_backward<T>(cp, le, sh, out);
+    // next layer's loads: in flight during the window sums and the flush
+    __builtin_amdgcn_sched_barrier(0);
+    if (kz + 1 < tb.nzt) {
+      load_plane_nodes<T, kMask>(r, xc, u, bcn, ncol + a.XY * (2 * ek + 2 + cp), a.X);
+      load_plane_eta<T>(r, eta + (long)(9 * cp) * a.nel + ecol + nlay * (ek + 1), a.nel);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // window sums, as in a11_tile_kernel
+    {
+      T *w = win[(2 * kz + cp) % 3] + 3 * ((2 * ly) * wX + 2 * lx);
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          if (live) {
+            T *q = w + 3 * (j * wX + i);
+            const T q0 = q[0], q1 = q[1], q2 = q[2];
+            q[0] = q0 + out[0][3 * j + i];
+            q[1] = q1 + out[1][3 * j + i];
+            q[2] = q2 + out[2][3 * j + i];
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __syncthreads();
+    flush_plane<T>(a, tile, tb, (int)(2 * ek), win[(2 * kz) % 3], y, fbx, fby, fbz);
+    flush_plane<T>(a, tile, tb, (int)(2 * ek + 1), win[(2 * kz + 1) % 3], y, fbx, fby, fbz);
+    __syncthreads();
+  }
+  flush_plane<T>(a, tile, tb, 2 * (tb.z0 + tb.nzt), win[(2 * tb.nzt) % 3], y, fbx, fby, fbz);
+}
+
+// ---------------------------------------------------------------------------
+// Partial assembly (kernel variant 2): per quadrature point the 9 entries of
+// Ah = sqrt(2 w eta / det) * adj(J) (adj(J) = det * J^-1, rows k: d xi / d x_k
+// scaled), stored once per coefficient/tiling change.  The pointwise action
+// of MFStokesWrapper_A11_Tensor (QuadratureAction, tensor.c:127-179) is then
+//   dv = sym(Gu Ah^T) Ah,   Gu[j][l] = d u_l / d xi_j,
+// which equals w detJ (2 eta sym(grad u)) G^T with G = J^-1.  Layout follows
+// the MatMult tiles so a lane group reads 21 consecutive values:
+//   pa[((((tile * ZC + kz) * 27 + q) * 9 + m) * 21) + le].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long pa_index(long tile, int ZC, int kz, int q, int m, int le) {
+  return ((((tile * ZC + kz) * 27 + q) * 9 + m) * (long)kEPW) + le;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    pa_build_kernel(TileArgs a, long ntiles, const double *__restrict__ xc,
+                    const double *__restrict__ eta, T *__restrict__ pa, int *__restrict__ bad) {
+  const long total = ntiles * a.ZC * 27L * kEPW;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int le = (int)(t % kEPW);
+    const int q = (int)((t / kEPW) % 27);
+    const int kz = (int)((t / (27L * kEPW)) % a.ZC);
+    const long tile = t / (27L * kEPW * a.ZC);
+    const TileBox tb = tile_box(a, tile);
+    if (kz >= tb.nzt) continue;
+    const int lx = le % a.TX, ly = le / a.TX;
+    const int ex = tb.x0 + min(lx, tb.nxt - 1), ey = tb.y0 + min(ly, tb.nyt - 1);
+    const long ek = tb.z0 + kz;
+    const long e = ex + (long)a.mx * (ey + (long)a.my * ek);
+    const int qa = q % 3, qb = (q / 3) % 3, qc = q / 9;
+    // M[j][d] = d x_d / d xi_j at the quadrature point
+    double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const long base = 2L * ex + a.X * (2L * ey) + a.XY * (2L * ek);
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+          const long n = base + i + a.X * j + a.XY * k;
+          const double g0 = kD[qa][i] * kB[qb][j] * kB[qc][k];
+          const double g1 = kB[qa][i] * kD[qb][j] * kB[qc][k];
+          const double g2 = kB[qa][i] * kB[qb][j] * kD[qc][k];
+          for (int d = 0; d < 3; d++) {
+            const double x = xc[3 * n + d];
+            M[0][d] += g0 * x;
+            M[1][d] += g1 * x;
+            M[2][d] += g2 * x;
+          }
+        }
+    double A[3][3];
+    A[0][0] = M[1][1] * M[2][2] - M[2][1] * M[1][2];
+    A[0][1] = M[2][1] * M[0][2] - M[0][1] * M[2][2];
+    A[0][2] = M[0][1] * M[1][2] - M[1][1] * M[0][2];
+    A[1][0] = M[2][0] * M[1][2] - M[1][0] * M[2][2];
+    A[1][1] = M[0][0] * M[2][2] - M[2][0] * M[0][2];
+    A[1][2] = M[1][0] * M[0][2] - M[0][0] * M[1][2];
+    A[2][0] = M[1][0] * M[2][1] - M[2][0] * M[1][1];
+    A[2][1] = M[2][0] * M[0][1] - M[0][0] * M[2][1];
+    A[2][2] = M[0][0] * M[1][1] - M[1][0] * M[0][1];
+    const double det = M[0][0] * A[0][0] + M[0][1] * A[1][0] + M[0][2] * A[2][0];
+    const double s = 2.0 * kW[qa] * kW[qb] * kW[qc] * eta[(long)q * a.nel + e] / det;
+    if (!(s > 0.0) || !isfinite(s)) *bad = 1;
+    const double r = sqrt(fabs(s));
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++) pa[pa_index(tile, a.ZC, kz, q, 3 * k + j, le)] = (T)(r * A[k][j]);
+  }
+}
+
+// Pointwise stage with stored geometry (see above); same exchange protocol
+// as plane_qp with the velocity in slots 0..26 of a 27-slot buffer: on
+// return slots 9l+p hold the zeta components, W the xi/eta accumulations.
+template <typename T>
+__device__ __forceinline__ void plane_qp_pa(int cp, int le, bool writer, const T Uq[3][9],
+                                            const T *__restrict__ pal, T (*sh)[27][kEPW],
+                                            T W[3][9]) {
+  T dz[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) dz[k] = sel3<T>(cp, kD[0][k], kD[1][k], kD[2][k]);
+#pragma unroll
+  for (int l = 0; l < 3; l++)
+#pragma unroll
+    for (int p = 0; p < 9; p++) W[l][p] = T(0);
+  T An[9];
+#pragma unroll
+  for (int m = 0; m < 9; m++) An[m] = pal[m * kEPW];
+#pragma unroll
+  for (int b = 0; b < 3; b++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const int p = 3 * b + a;
+      T Ah[9];
+#pragma unroll
+      for (int m = 0; m < 9; m++) Ah[m] = An[m];
+      if (p < 8) {
+#pragma unroll
+        for (int m = 0; m < 9; m++) An[m] = pal[((p + 1) * 9 + m) * kEPW];
+      }
+      T Gu[3][3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        Gu[0][d] = (T)kC[a][0] * Uq[d][3 * b] + (T)kC[a][1] * Uq[d][3 * b + 1] +
+                   (T)kC[a][2] * Uq[d][3 * b + 2];
+        Gu[1][d] = (T)kC[b][0] * Uq[d][a] + (T)kC[b][1] * Uq[d][3 + a] + (T)kC[b][2] * Uq[d][6 + a];
+        Gu[2][d] = dz[0] * sh[0][9 * d + p][le] + dz[1] * sh[1][9 * d + p][le] +
+                   dz[2] * sh[2][9 * d + p][le];
+      }
+      // gh[k][l] = sum_j Gu[j][l] Ah[k][j]; sg = sym(gh)
+      T gh[3][3];
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+          gh[k][l] = Gu[0][l] * Ah[3 * k] + Gu[1][l] * Ah[3 * k + 1] + Gu[2][l] * Ah[3 * k + 2];
+      T sg[3][3];
+      sg[0][0] = gh[0][0];
+      sg[1][1] = gh[1][1];
+      sg[2][2] = gh[2][2];
+      sg[0][1] = sg[1][0] = T(0.5) * (gh[0][1] + gh[1][0]);
+      sg[0][2] = sg[2][0] = T(0.5) * (gh[0][2] + gh[2][0]);
+      sg[1][2] = sg[2][1] = T(0.5) * (gh[1][2] + gh[2][1]);
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        const T dv0 = sg[0][l] * Ah[0] + sg[1][l] * Ah[3] + sg[2][l] * Ah[6];
+        const T dv1 = sg[0][l] * Ah[1] + sg[1][l] * Ah[4] + sg[2][l] * Ah[7];
+        const T dv2 = sg[0][l] * Ah[2] + sg[1][l] * Ah[5] + sg[2][l] * Ah[8];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          W[l][3 * b + r] += (T)kC[a][r] * dv0;
+          W[l][3 * r + a] += (T)kC[b][r] * dv1;
+        }
+        if (writer) sh[cp][9 * l + p][le] = dv2;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  __syncthreads();
+}
+
+// MatMult with stored geometry: the z-march of a11_tile_kernel with only the
+// velocity exchanged (27 LDS slots) and the backward zeta stage in two
+// passes over the same slots.
+template <typename T, bool kMask>
+__global__ void __launch_bounds__(64)
+    a11_tile_pa_kernel(TileArgs a, const T *__restrict__ pa, const uint8_t *__restrict__ bcn,
+                       const T *__restrict__ u, T *__restrict__ y, T *__restrict__ fbx,
+                       T *__restrict__ fby, T *__restrict__ fbz) {
+  __shared__ T sh[3][27][kEPW];
+  __shared__ T win[3][3 * kWinNodes];
+  const long tile = blockIdx.x;
+  const TileBox tb = tile_box(a, tile);
+  const int lane = threadIdx.x;
+  const int c = lane / kEPW;
+  const int le = lane - c * kEPW;
+  const int cp = c < 3 ? c : 2;
+  const bool writer = c < 3;
+  const int lx = le % a.TX, ly = le / a.TX;
+  const bool live = writer && (le < a.TX * a.TY) && (lx < tb.nxt) && (ly < tb.nyt);
+  const int ex = tb.x0 + min(lx, tb.nxt - 1), ey = tb.y0 + min(ly, tb.nyt - 1);
+  const int wX = 2 * tb.nxt + 1;
+  const long ncol = 2L * ex + a.X * (2L * ey);
+
+  for (int t = lane; t < 9 * kWinNodes; t += 64) (&win[0][0])[t] = T(0);
+  __syncthreads();
+
+  T bz[3], bt[3], dt[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    bz[k] = sel3<T>(cp, kB[0][k], kB[1][k], kB[2][k]);
+    bt[k] = sel3<T>(cp, kB[k][0], kB[k][1], kB[k][2]);
+    dt[k] = sel3<T>(cp, kD[k][0], kD[k][1], kD[k][2]);
+  }
+
+  for (int kz = 0; kz < tb.nzt; kz++) {
+    const long ek = tb.z0 + kz;
+    const long base = ncol + a.XY * (2 * ek + cp);
+    const T *pal = pa + pa_index(tile, a.ZC, kz, 9 * cp, 0, le);
+    // forward: masked velocity plane, in-plane interpolation, zeta exchange
+    T Uq[3][9];
+    {
+      T g[3][9];
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          const long node = base + i + a.X * j;
+          const unsigned m = kMask ? (unsigned)bcn[node] : 0u;
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            const T v = u[3 * node + d];
+            g[d][3 * j + i] = ((m >> d) & 1u) ? T(0) : v;
+          }
+        }
+      T V[3][9];
+      interp_plane<T>(g, V);
+      if (writer) {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+          for (int p = 0; p < 9; p++) sh[cp][9 * d + p][le] = V[d][p];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int p = 0; p < 9; p++)
+        Uq[d][p] = bz[0] * sh[0][9 * d + p][le] + bz[1] * sh[1][9 * d + p][le] +
+                   bz[2] * sh[2][9 * d + p][le];
+    T W[3][9];
+    plane_qp_pa<T>(cp, le, writer, Uq, pal, sh, W);
+    // backward zeta stage, pass 1: the zeta components
+    T Y[3][9];
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+      for (int p = 0; p < 9; p++)
+        Y[l][p] = dt[0] * sh[0][9 * l + p][le] + dt[1] * sh[1][9 * l + p][le] +
+                  dt[2] * sh[2][9 * l + p][le];
+    __syncthreads();
+    if (writer) {
+#pragma unroll
+      for (int l = 0; l < 3; l++)
+#pragma unroll
+        for (int p = 0; p < 9; p++) sh[cp][9 * l + p][le] = W[l][p];
+    }
+    __syncthreads();
+    // pass 2: the xi/eta accumulations
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+      for (int p = 0; p < 9; p++)
+        Y[l][p] += bt[0] * sh[0][9 * l + p][le] + bt[1] * sh[1][9 * l + p][le] +
+                   bt[2] * sh[2][9 * l + p][le];
+    __syncthreads();
+    T out[3][9];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      T t[3][3];
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+          t[b][i] = (T)kB[0][i] * Y[l][3 * b] + (T)kB[1][i] * Y[l][3 * b + 1] +
+                    (T)kB[2][i] * Y[l][3 * b + 2];
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+          out[l][3 * j + i] = (T)kB[0][j] * t[0][i] + (T)kB[1][j] * t[1][i] + (T)kB[2][j] * t[2][i];
+    }
+    {
+      T *w = win[(2 * kz + cp) % 3] + 3 * ((2 * ly) * wX + 2 * lx);
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          if (live) {
+            T *q = w + 3 * (j * wX + i);
+            const T q0 = q[0], q1 = q[1], q2 = q[2];
+            q[0] = q0 + out[0][3 * j + i];
+            q[1] = q1 + out[1][3 * j + i];
+            q[2] = q2 + out[2][3 * j + i];
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __syncthreads();
+    flush_plane<T>(a, tile, tb, (int)(2 * ek), win[(2 * kz) % 3], y, fbx, fby, fbz);
+    flush_plane<T>(a, tile, tb, (int)(2 * ek + 1), win[(2 * kz + 1) % 3], y, fbx, fby, fbz);
+    __syncthreads();
+  }
+  flush_plane<T>(a, tile, tb, 2 * (tb.z0 + tb.nzt), win[(2 * tb.nzt) % 3], y, fbx, fby, fbz);
+}
+
 // Adds the neighbours' face-buffer contributions into the nodes on the lower
 // faces of each tile.
 template <typename T>
@@ -515,8 +906,51 @@ int launch_a11(const ptmf_op_s *op, const T *u, T *y, bool mask_columns) {
   return PTMF_OK;
 }
 
+// (Re)build the stored geometry of variant 2 for precision T when the
+// coefficients or the tiling changed since the last build.
 template <typename T>
-int launch_a11_mult(const ptmf_op_s *op, const T *x, T *y, bool faces) {
+int ensure_pa(ptmf_op_s *op) {
+  const int ip = sizeof(T) == 8 ? 0 : 1;
+  const Tiling &tl = op->tiling;
+  if (op->pa_state[ip] == op->state && op->pa_tiling[ip][0] == tl.TX &&
+      op->pa_tiling[ip][1] == tl.TY && op->pa_tiling[ip][2] == tl.ZC)
+    return PTMF_OK;
+  const long len = tl.ntiles * tl.ZC * 27L * 9L * kEPW;
+  if (len > op->pa_len) {
+    PTMF_HIP(hipStreamSynchronize(op->stream));
+    (void)hipFree(op->pa64);
+    (void)hipFree(op->pa32);
+    op->pa64 = nullptr;
+    op->pa32 = nullptr;
+    op->pa_state[0] = op->pa_state[1] = ~0UL;
+    op->pa_len = len;
+  }
+  T **dst = (T **)(sizeof(T) == 8 ? (void *)&op->pa64 : (void *)&op->pa32);
+  if (!*dst) PTMF_HIP(hipMalloc((void **)dst, sizeof(T) * op->pa_len));
+  if (!op->pa_bad) PTMF_HIP(hipMalloc((void **)&op->pa_bad, sizeof(int)));
+  PTMF_TRY(launch_fill<int>(op->pa_bad, 0, 1, op->stream));
+  const TileArgs a = tile_args(op);
+  const long total = tl.ntiles * tl.ZC * 27L * kEPW;
+  long nb = (total + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  pa_build_kernel<T><<<dim3((unsigned)nb), dim3(256), 0, op->stream>>>(a, tl.ntiles, op->d64.coords,
+                                                                      op->d64.eta, *dst, op->pa_bad);
+  PTMF_HIP(hipGetLastError());
+  int bad = 0;
+  PTMF_HIP(hipMemcpyAsync(&bad, op->pa_bad, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  PTMF_CHECK(!bad, PTMF_ERR_ARG,
+             "a11 (stored geometry): non-positive or non-finite 2 w eta / det at some quadrature "
+             "point (inverted element or eta <= 0)");
+  op->pa_state[ip] = op->state;
+  op->pa_tiling[ip][0] = tl.TX;
+  op->pa_tiling[ip][1] = tl.TY;
+  op->pa_tiling[ip][2] = tl.ZC;
+  return PTMF_OK;
+}
+
+template <typename T>
+int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, bool faces) {
   const OpData<T> &d = *(const OpData<T> *)(sizeof(T) == 8 ? (const void *)&op->d64
                                                            : (const void *)&op->d32);
   const Tiling &tl = op->tiling;
@@ -527,8 +961,17 @@ int launch_a11_mult(const ptmf_op_s *op, const T *x, T *y, bool faces) {
   T *fb = (T *)op->faces;
   const long ntiles = tl.ntiles;
   T *fbx = fb, *fby = fbx + 3 * a.fbx_n * ntiles, *fbz = fby + 3 * a.fby_n * ntiles;
-  a11_tile_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
-      a, d.coords, d.eta, op->bcnode, x, y, fbx, fby, fbz);
+  if (op->variant == 2) {
+    PTMF_TRY(ensure_pa<T>(op));
+    const T *pa = sizeof(T) == 8 ? (const T *)op->pa64 : (const T *)op->pa32;
+    a11_tile_pa_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
+        a, pa, op->bcnode, x, y, fbx, fby, fbz);
+  } else if (op->variant == 1)
+    a11_tile_pf_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
+        a, d.coords, d.eta, op->bcnode, x, y, fbx, fby, fbz);
+  else
+    a11_tile_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
+        a, d.coords, d.eta, op->bcnode, x, y, fbx, fby, fbz);
   PTMF_HIP(hipGetLastError());
   if (faces) {
     a11_face_kernel<T><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(a, fbx, fby, fbz, y);
@@ -540,7 +983,7 @@ int launch_a11_mult(const ptmf_op_s *op, const T *x, T *y, bool faces) {
 
 template int launch_a11<double>(const ptmf_op_s *, const double *, double *, bool);
 template int launch_a11<float>(const ptmf_op_s *, const float *, float *, bool);
-template int launch_a11_mult<double>(const ptmf_op_s *, const double *, double *, bool);
-template int launch_a11_mult<float>(const ptmf_op_s *, const float *, float *, bool);
+template int launch_a11_mult<double>(ptmf_op_s *, const double *, double *, bool);
+template int launch_a11_mult<float>(ptmf_op_s *, const float *, float *, bool);
 
 }  // namespace ptmf

@@ -109,6 +109,14 @@ int ptmf_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   return PTMF_OK;
 }
 
+int ptmf_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+  PTMF_CHECK(bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0, PTMF_ERR_ARG,
+             "memcpy_dtod: 16-byte aligned pointers and sizes only");
+  PTMF_TRY(ptmf::launch_copy16(src, dst, (long)(bytes / 16), 0));
+  PTMF_HIP(hipStreamSynchronize(0));
+  return PTMF_OK;
+}
+
 int ptmf_memset(void *dptr, int value, size_t bytes) {
   if (bytes) PTMF_HIP(hipMemset(dptr, value, bytes));
   return PTMF_OK;
@@ -173,6 +181,9 @@ int ptmf_op_destroy(ptmf_op *pop) {
   (void)hipFree(op->faces);
   (void)hipFree(op->host_stage);
   (void)hipFree(op->host_stage2);
+  (void)hipFree(op->pa64);
+  (void)hipFree(op->pa32);
+  (void)hipFree(op->pa_bad);
   if (op->ev0) (void)hipEventDestroy(op->ev0);
   if (op->ev1) (void)hipEventDestroy(op->ev1);
   if (op->stream) (void)hipStreamDestroy(op->stream);
@@ -204,6 +215,20 @@ int ptmf_op_set_tiling(ptmf_op op, int TX, int TY, int ZC) {
     op->faces_doubles = need;
   }
   op->tiling = t;
+  return PTMF_OK;
+}
+
+int ptmf_op_set_kernel_variant(ptmf_op op, int variant) {
+  PTMF_CHECK(op, PTMF_ERR_ARG, "set_kernel_variant: null operator");
+  PTMF_CHECK(variant >= 0 && variant <= 2, PTMF_ERR_UNSUPPORTED,
+             "set_kernel_variant: unknown variant %d", variant);
+  op->variant = variant;
+  return PTMF_OK;
+}
+
+int ptmf_op_get_kernel_variant(ptmf_op op, int *variant) {
+  PTMF_CHECK(op && variant, PTMF_ERR_ARG, "get_kernel_variant: null argument");
+  *variant = op->variant;
   return PTMF_OK;
 }
 

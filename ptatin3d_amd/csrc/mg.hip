@@ -34,7 +34,7 @@ namespace {
 
 constexpr int kBS = 256;
 constexpr int kRedBlocks = 1024;
-constexpr int kMaxDots = 64;
+constexpr int kMaxDots = 512;
 
 inline unsigned nblocks(long n) {
   long g = (n + kBS - 1) / kBS;
@@ -408,6 +408,7 @@ struct ptmf_mg_s {
   int restart = 0;
   std::vector<double *> V, Z;
   double *w = nullptr, *pcg[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *fcg[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // flexible CG workspace
   std::vector<double> history;
 };
 
@@ -864,6 +865,7 @@ int ptmf_mg_destroy(ptmf_mg *pmg) {
   for (double *p : mg->Z) (void)hipFree(p);
   (void)hipFree(mg->w);
   for (double *p : mg->pcg) (void)hipFree(p);
+  for (double *p : mg->fcg) (void)hipFree(p);
   (void)hipFree(mg->red_part);
   (void)hipFree(mg->red_out);
   (void)hipFree(mg->dV);
@@ -1192,6 +1194,79 @@ int ptmf_mg_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxi
     PTMF_HIP(hipMemcpyAsync(mg->coef_dev, mg->coef_host, sizeof(double) * j, hipMemcpyHostToDevice, s));
     ptmf::k_mupdate<<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, j, mg->dZ, mg->coef_dev, x);
     PTMF_HIP(hipGetLastError());
+  }
+  PTMF_HIP(hipStreamSynchronize(s));
+  if (its_out) *its_out = its;
+  if (rnorm_out) *rnorm_out = rnorm;
+  return PTMF_OK;
+}
+
+
+// Flexible conjugate gradients preconditioned by the V-cycle (Polak-Ribiere
+// beta = z_k.(r_k - r_{k-1}) / (z_{k-1}.r_{k-1}), robust to the small
+// nonlinearity of an FP32 V-cycle).  Starts from the given x; converged when
+// the FP64 recursive residual drops below rtol * ||b||, after which the true
+// residual is recomputed (and iteration resumes if it is not below rtol).
+int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int *its_out,
+                double *rnorm_out, double *history, int history_len) {
+  PTMF_CHECK(mg && b && x && maxit > 0, PTMF_ERR_ARG, "mg_fcg: bad arguments");
+  PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_fcg: not set up");
+  ptmf::MGLevel &F = mg->lev[mg->L - 1];
+  const long n = F.n;
+  hipStream_t s = mg->stream;
+  for (auto &p : mg->fcg)
+    if (!p) PTMF_TRY(ptmf::dalloc((void **)&p, sizeof(double) * n));
+  double *r = mg->fcg[0], *z = mg->fcg[1], *p = mg->fcg[2], *q = mg->fcg[3], *rold = mg->fcg[4];
+  double bb;
+  PTMF_TRY(ptmf::dot(mg, n, b, b, &bb));
+  const double bnorm = sqrt(bb);
+  int its = 0, hl = 0;
+  double rnorm = 0.0;
+  if (bnorm == 0.0) {
+    PTMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    PTMF_HIP(hipStreamSynchronize(s));
+    if (its_out) *its_out = 0;
+    if (rnorm_out) *rnorm_out = 0.0;
+    return PTMF_OK;
+  }
+  while (true) {
+    // true residual
+    PTMF_TRY(ptmf::matmult<double>(F, x, q));
+    ptmf::k_residual<double><<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, b, q, r);
+    PTMF_HIP(hipGetLastError());
+    double rr;
+    PTMF_TRY(ptmf::dot(mg, n, r, r, &rr));
+    rnorm = sqrt(rr);
+    PTMF_CHECK(isfinite(rnorm), PTMF_ERR_DIVERGED, "fcg: non-finite residual");
+    if (history && hl < history_len) history[hl++] = rnorm;
+    if (rnorm <= rtol * bnorm || its >= maxit) break;
+    PTMF_TRY(ptmf_mg_vcycle(mg, r, z));
+    PTMF_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    double rz;
+    PTMF_TRY(ptmf::dot(mg, n, r, z, &rz));
+    while (its < maxit) {
+      PTMF_TRY(ptmf::matmult<double>(F, p, q));
+      double pq;
+      PTMF_TRY(ptmf::dot(mg, n, p, q, &pq));
+      PTMF_CHECK(isfinite(pq) && pq != 0.0, PTMF_ERR_DIVERGED, "fcg: breakdown (p.Ap = %g)", pq);
+      const double alpha = rz / pq;
+      PTMF_TRY(ptmf::axpby<double>(s, n, alpha, p, 1.0, x));
+      PTMF_HIP(hipMemcpyAsync(rold, r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      PTMF_TRY(ptmf::axpby<double>(s, n, -alpha, q, 1.0, r));
+      PTMF_TRY(ptmf::dot(mg, n, r, r, &rr));
+      its++;
+      rnorm = sqrt(rr);
+      PTMF_CHECK(isfinite(rnorm), PTMF_ERR_DIVERGED, "fcg: non-finite residual");
+      if (history && hl < history_len) history[hl++] = rnorm;
+      if (rnorm <= rtol * bnorm) break;
+      PTMF_TRY(ptmf_mg_vcycle(mg, r, z));
+      double zr, zro;
+      PTMF_TRY(ptmf::dot(mg, n, z, r, &zr));
+      PTMF_TRY(ptmf::dot(mg, n, z, rold, &zro));
+      const double beta = (zr - zro) / rz;
+      rz = zr;
+      PTMF_TRY(ptmf::axpby<double>(s, n, 1.0, z, beta, p));
+    }
   }
   PTMF_HIP(hipStreamSynchronize(s));
   if (its_out) *its_out = its;

@@ -85,6 +85,15 @@ struct ptmf_op_s {
   hipStream_t stream = nullptr;   // stream all work of this operator runs on
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ptmf::Tiling tiling;
+  int variant = 0;  // MatMult kernel: 0 z-march, 1 software-pipelined, 2 stored geometry
+  // stored quadrature-point geometry of variant 2 (per precision), valid for
+  // coefficient state pa_state[i] and the tiling it was laid out for
+  double *pa64 = nullptr;
+  float *pa32 = nullptr;
+  long pa_len = 0;
+  unsigned long pa_state[2] = {~0UL, ~0UL};
+  int pa_tiling[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  int *pa_bad = nullptr;
   void *faces = nullptr;          // tile face buffers (sized for FP64)
   long faces_doubles = 0;
   double *host_stage = nullptr;   // device staging for the host-array entry points
@@ -121,13 +130,14 @@ int launch_a11(const ptmf_op_s *op, const T *u, T *y, bool mask_columns);
 // MatMult_MFStokes_A11: y = A11 mask(x) on free rows, y = x on Dirichlet rows.
 // faces == false runs the element (tile) kernel alone, for timing.
 template <typename T>
-int launch_a11_mult(const ptmf_op_s *op, const T *x, T *y, bool faces);
+int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, bool faces);
 template <typename T>
 int launch_fill(T *x, T v, long n, hipStream_t s);
 template <typename T>
 int launch_bc_copy(const long *idx, long n, const T *x, T *y, hipStream_t s);
 template <typename T>
 int launch_bc_set(const long *idx, long n, T v, T *y, hipStream_t s);
+int launch_copy16(const void *x, void *y, long n16, hipStream_t s);
 int launch_a11_diag(const ptmf_op_s *op, double *d);
 int launch_a12(const ptmf_op_s *op, const double *p, double *yu);
 int launch_a21(const ptmf_op_s *op, const double *u, double *yp);

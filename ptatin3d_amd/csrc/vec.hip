@@ -28,6 +28,12 @@ __global__ void bc_set_kernel(const long *__restrict__ idx, long n, T v, T *__re
     y[idx[k]] = v;
 }
 
+// plain 16-byte-vector copy (also the reference kernel for HBM counters)
+__global__ void __launch_bounds__(256) copy16_kernel(const uint4 *__restrict__ x, uint4 *__restrict__ y, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+
 inline unsigned grid_for(long n, int bs) {
   long g = (n + bs - 1) / bs;
   if (g > 65536) g = 65536;
@@ -61,8 +67,16 @@ int launch_bc_set(const long *idx, long n, T v, T *y, hipStream_t s) {
   return PTMF_OK;
 }
 
+int launch_copy16(const void *x, void *y, long n16, hipStream_t s) {
+  if (n16 <= 0) return PTMF_OK;
+  copy16_kernel<<<grid_for(n16, 256), 256, 0, s>>>((const uint4 *)x, (uint4 *)y, n16);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
 template int launch_fill<double>(double *, double, long, hipStream_t);
 template int launch_fill<float>(float *, float, long, hipStream_t);
+template int launch_fill<int>(int *, int, long, hipStream_t);
 template int launch_bc_copy<double>(const long *, long, const double *, double *, hipStream_t);
 template int launch_bc_copy<float>(const long *, long, const float *, float *, hipStream_t);
 template int launch_bc_set<double>(const long *, long, double, double *, hipStream_t);

@@ -18,19 +18,21 @@ def nnodes(mx: int, my: int, mz: int) -> int:
 def box_coords(mx, my, mz, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)) -> np.ndarray:
     """Affine box, node-interlaced coordinates (3N)."""
     X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
-    xs = np.linspace(lo[0], hi[0], X)
-    ys = np.linspace(lo[1], hi[1], Y)
-    zs = np.linspace(lo[2], hi[2], Z)
-    zz, yy, xx = np.meshgrid(zs, ys, xs, indexing="ij")
-    return np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1).reshape(-1).copy()
+    out = np.empty((Z, Y, X, 3))
+    out[..., 0] = np.linspace(lo[0], hi[0], X)[None, None, :]
+    out[..., 1] = np.linspace(lo[1], hi[1], Y)[None, :, None]
+    out[..., 2] = np.linspace(lo[2], hi[2], Z)[:, None, None]
+    return out.reshape(-1)
 
 
 def boundary_node_flags(mx, my, mz) -> np.ndarray:
     """Boolean per node: lies on any face of the box."""
     X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
-    k, j, i = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
-    b = (i == 0) | (i == X - 1) | (j == 0) | (j == Y - 1) | (k == 0) | (k == Z - 1)
-    return b.ravel()
+    b = np.zeros((Z, Y, X), dtype=bool)
+    b[0] = b[-1] = True
+    b[:, 0] = b[:, -1] = True
+    b[:, :, 0] = b[:, :, -1] = True
+    return b.reshape(-1)
 
 
 def perturb_interior(coords, mx, my, mz, frac=0.1, rng=None) -> np.ndarray:
@@ -118,11 +120,55 @@ def smooth_body_force(coords, mx, my, mz, rng=None) -> np.ndarray:
     rng = rng or np.random.default_rng(SEED + 5)
     a = rng.uniform(0.5, 1.5, size=3)
     c = coords.reshape(-1, 3)
-    s = np.sin(np.pi * c)
-    f = np.stack([a[0] * s[:, 0] * s[:, 1], a[1] * s[:, 1] * s[:, 2], a[2] * s[:, 2] * s[:, 0]], axis=1)
-    f = f.reshape(-1) / (8.0 * mx * my * mz)
+    f = np.empty_like(c)
+    sx, sy, sz = (np.sin(np.pi * c[:, d]) for d in range(3))
+    np.multiply(sx, sy, out=f[:, 0])
+    np.multiply(sy, sz, out=f[:, 1])
+    np.multiply(sz, sx, out=f[:, 2])
+    f *= a[None, :] / (8.0 * mx * my * mz)
+    f = f.reshape(-1)
     f[dirichlet_all_faces(mx, my, mz).astype(bool)] = 0.0
     return f
+
+
+def box_qp_axes(mx, my, mz):
+    """Quadrature-point positions of the affine unit-cube box, per axis:
+    gx[ei, a] = (ei + (1 + g_a) / 2) / mx, likewise gy, gz."""
+    g = 0.5 * (1.0 + GAUSS)
+    return tuple((np.arange(m)[:, None] + g[None, :]) / m for m in (mx, my, mz))
+
+
+def _ranges(ax, c, r):
+    """Element index range whose quadrature points may lie within [c-r, c+r]."""
+    m = ax.shape[0]
+    lo = int(np.clip(np.floor((c - r) * m) - 1, 0, m))
+    hi = int(np.clip(np.ceil((c + r) * m) + 1, 0, m))
+    return lo, hi
+
+
+def eta_layered_ellipsoids_box(mx, my, mz, n_ell=64, rng=None) -> np.ndarray:
+    """eta_layered_ellipsoids on the affine unit-cube box, evaluated per
+    ellipsoid over its bounding box only (same seeded draws)."""
+    rng = rng or np.random.default_rng(SEED + 3)
+    gx, gy, gz = box_qp_axes(mx, my, mz)
+    # element-major [27e + q] = array (ek, ej, ei, c, b, a)
+    zl = np.where(gz < 1.0 / 3.0, 1e3, np.where(gz < 2.0 / 3.0, 1e1, 1e-3))  # (mz, 3)
+    eta = np.empty((mz, my, mx, 3, 3, 3))
+    eta[...] = zl[:, None, None, :, None, None]
+    cen = rng.uniform(0.0, 1.0, size=(n_ell, 3))
+    axes = rng.uniform(0.03, 0.1, size=(n_ell, 3))
+    for c, a in zip(cen, axes):
+        i0, i1 = _ranges(gx, c[0], a[0])
+        j0, j1 = _ranges(gy, c[1], a[1])
+        k0, k1 = _ranges(gz, c[2], a[2])
+        dx = ((gx[i0:i1] - c[0]) / a[0]) ** 2  # (ni, 3a)
+        dy = ((gy[j0:j1] - c[1]) / a[1]) ** 2
+        dz = ((gz[k0:k1] - c[2]) / a[2]) ** 2
+        r2 = (dz[:, None, None, :, None, None] + dy[None, :, None, None, :, None]
+              + dx[None, None, :, None, None, :])
+        sub = eta[k0:k1, j0:j1, i0:i1]
+        sub[r2 <= 1.0] = 1e-3
+    return eta.reshape(-1)
 
 
 def random_velocity(mx, my, mz, bcmask=None, rng=None) -> np.ndarray:
@@ -167,7 +213,7 @@ def config3(m=128, seed=SEED):
     (contrast 1e6), smooth body-force load."""
     rng = np.random.default_rng(seed)
     coords = box_coords(m, m, m)
-    eta = eta_layered_ellipsoids(qp_coordinates(coords, m, m, m), rng=rng)
+    eta = eta_layered_ellipsoids_box(m, m, m, rng=rng)
     bc = dirichlet_all_faces(m, m, m)
     b = smooth_body_force(coords, m, m, m, rng)
     return dict(mx=m, my=m, mz=m, coords=coords, eta=eta, bc=bc, b=b)

@@ -117,6 +117,16 @@ class GMG:
         nz = np.flatnonzero(hist)
         return its.value, rn.value, hist[: (nz[-1] + 1) if nz.size else 0]
 
+    def fcg(self, b: DeviceArray, x: DeviceArray, rtol=1e-8, maxit=500, history_len=1024):
+        """Flexible CG preconditioned by the V-cycle, from the given x."""
+        its = C.c_int(0)
+        rn = C.c_double(0)
+        hist = np.zeros(history_len)
+        call("ptmf_mg_fcg", self.handle, b.ptr, x.ptr, float(rtol), int(maxit),
+             C.byref(its), C.byref(rn), hist.ctypes.data_as(C.POINTER(C.c_double)), int(history_len))
+        nz = np.flatnonzero(hist)
+        return its.value, rn.value, hist[: (nz[-1] + 1) if nz.size else 0]
+
     def destroy(self) -> None:
         if self.handle is not None and self.handle.value:
             call("ptmf_mg_destroy", C.byref(self.handle))
@@ -130,14 +140,18 @@ class GMG:
             pass
 
 
-def solve_timed(mg: GMG, b: np.ndarray, rtol=1e-8, maxit=200, restart=30):
-    """Upload b, run FGMRES from x = 0; returns (x, its, rnorm, history, seconds)."""
+def solve_timed(mg: GMG, b: np.ndarray, rtol=1e-8, maxit=200, restart=30, krylov="fgmres"):
+    """Upload b, run FGMRES (or flexible CG) from x = 0; returns (x, its,
+    rnorm, history, seconds)."""
     db = DeviceArray.from_host(b)
     dx = DeviceArray(b.size)
     dx.zero()
     call("ptmf_device_synchronize")
     t0 = time.perf_counter()
-    its, rn, hist = mg.fgmres(db, dx, rtol, maxit, restart)
+    if krylov == "fcg":
+        its, rn, hist = mg.fcg(db, dx, rtol, maxit)
+    else:
+        its, rn, hist = mg.fgmres(db, dx, rtol, maxit, restart)
     call("ptmf_device_synchronize")
     t1 = time.perf_counter()
     x = dx.download()

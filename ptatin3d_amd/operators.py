@@ -66,6 +66,15 @@ class MatA11MF:
         call("ptmf_op_get_tiling", self.handle, C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
 
+    def set_kernel_variant(self, variant: int) -> None:
+        """0: z-march; 1: z-march with next-layer loads issued one stage ahead."""
+        call("ptmf_op_set_kernel_variant", self.handle, int(variant))
+
+    def get_kernel_variant(self) -> int:
+        v = C.c_int()
+        call("ptmf_op_get_kernel_variant", self.handle, C.byref(v))
+        return v.value
+
     @property
     def state(self) -> int:
         s = C.c_ulong(0)

@@ -70,6 +70,16 @@ def test_tilings(gpu, tiling):
     y = A.mult_host(x)
     yo = O.a11_matmult(*m, coords, eta, bc, x)
     assert rel_inf(y, yo) <= TOL64
+    y32 = A.mult_host(x, precision=32)
+    assert rel_inf(y32, yo) <= TOL32
+    # the software-pipelined kernel sums in the same order: identical results
+    A.set_kernel_variant(1)
+    assert A.get_kernel_variant() == 1
+    assert np.array_equal(A.mult_host(x), y)
+    assert np.array_equal(A.mult_host(x, precision=32), y32)
+    # stored quadrature-point geometry: the same operator to rounding
+    A.set_kernel_variant(2)
+    assert rel_inf(A.mult_host(x), yo) <= TOL64
     assert rel_inf(A.mult_host(x, precision=32), yo) <= TOL32
     A.destroy()
 
