@@ -112,7 +112,7 @@ int ptmf_op_set_viscosity_device(ptmf_op op, const double *eta);
 int ptmf_op_set_tiling(ptmf_op op, int TX, int TY, int ZC);
 int ptmf_op_get_tiling(ptmf_op op, int *TX, int *TY, int *ZC);
 /* MatMult kernel variant: 0 = z-march; 1 = z-march with the next layer's
- * loads issued one stage ahead (software pipelined; bit-identical to 0);
+ * loads issued one stage ahead (software pipelined; equal to 0 to rounding);
  * 2 = stored quadrature-point geometry (partial assembly: 9 values per
  * quadrature point, rebuilt after a coefficient or tiling change; same
  * operator to rounding, needs positive 2 w eta / det) */

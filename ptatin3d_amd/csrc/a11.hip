@@ -79,9 +79,9 @@ __device__ __forceinline__ void load_plane_nodes(PlaneRaw<T> &r, const T *__rest
 }
 
 template <typename T>
-__device__ __forceinline__ void load_plane_eta(PlaneRaw<T> &r, const T *__restrict__ etap, long nel) {
+__device__ __forceinline__ void load_plane_eta(PlaneRaw<T> &r, const T *__restrict__ etap, long stride) {
 #pragma unroll
-  for (int p = 0; p < 9; p++) r.eta[p] = etap[(long)p * nel];
+  for (int p = 0; p < 9; p++) r.eta[p] = etap[(long)p * stride];
 }
 
 // in-plane interpolation of a 3x3 node plane to the 3x3 quadrature grid:
@@ -297,15 +297,15 @@ __device__ __forceinline__ void plane_backward(int cp, int le, T (*sh)[kSlots][k
 // Every lane of the wave must call it (it contains wave-wide barriers); lanes
 // with writer == false (lane 63) never write the exchange buffer.
 template <typename T, bool kMask>
-__device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer, long e, long nel,
-                                                    long base, long X, const T *__restrict__ xc,
-                                                    const T *__restrict__ eta,
+__device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer, const T *__restrict__ etap,
+                                                    long estride, long base, long X,
+                                                    const T *__restrict__ xc,
                                                     const uint8_t *__restrict__ bcn,
                                                     const T *__restrict__ u,
                                                     T (*sh)[kSlots][kEPW], T out[3][9]) {
   PlaneRaw<T> r;
   load_plane_nodes<T, kMask>(r, xc, u, bcn, base, X);
-  load_plane_eta<T>(r, eta + (long)(9 * cp) * nel + e, nel);
+  load_plane_eta<T>(r, etap, estride);
   T Xq[3][9], Uq[3][9];
   plane_forward<T, kMask>(cp, le, writer, r, sh, Xq, Uq);
   plane_qp<T>(cp, le, writer, Xq, Uq, r.eta, sh);
@@ -331,7 +331,8 @@ __global__ void __launch_bounds__(64)
   const long ei = e % mx, er = e / mx, ej = er % my, ek = er / my;
   const long base = 2 * ei + X * (2 * ej) + XY * (2 * ek + cp);
   T out[3][9];
-  element_plane_apply<T, kMask>(cp, le, c < 3, e, nel, base, X, xc, eta, bcn, u, sh, out);
+  element_plane_apply<T, kMask>(cp, le, c < 3, eta + (long)(9 * cp) * nel + e, nel, base, X, xc, bcn,
+                                u, sh, out);
   if (live) {
 #pragma unroll
     for (int l = 0; l < 3; l++)
@@ -349,8 +350,18 @@ struct TileArgs {
   int mx, my, mz;
   long X, XY, nel;
   int TX, TY, ZC, ntx, nty;
+  long ntiles;
   long fbx_n, fby_n, fbz_n;  // per-tile face buffer sizes in nodes
 };
+
+// Work-groups are dispatched round-robin over the 8 XCDs (each with its own
+// L2); give every XCD a contiguous run of tiles (x fastest), so tiles that
+// share halo nodes and viscosity cache lines share an L2.
+__device__ __forceinline__ long xcd_tile(long b, long ntiles) {
+  const long per = ntiles / 8, rem = ntiles % 8;
+  const long x = b % 8, slot = b / 8;
+  return x * per + (x < rem ? x : rem) + slot;
+}
 
 struct TileBox {
   int tx, ty, tz, x0, y0, z0, nxt, nyt, nzt;
@@ -372,6 +383,33 @@ __device__ __forceinline__ TileBox tile_box(const TileArgs &a, long tile) {
   t.yend = t.y0 + t.nyt == a.my;
   t.zend = t.z0 + t.nzt == a.mz;
   return t;
+}
+
+// Tile-ordered quadrature-point data (viscosity for variants 0/1): for tile
+// t, layer kz of its z-chunk, quadrature point q and element slot le of the
+// wave, index ((t * ZC + kz) * 27 + q) * 21 + le, so the lanes of a plane read
+// 21 consecutive values.  Slots beyond a partial tile repeat the clamped
+// element, as the MatMult kernels do.
+__device__ __forceinline__ long tq_index(long tile, int ZC, int kz, int q, int le) {
+  return ((tile * ZC + kz) * 27 + q) * (long)kEPW + le;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    eta_tile_kernel(TileArgs a, const double *__restrict__ eta, T *__restrict__ et) {
+  const long total = a.ntiles * a.ZC * 27L * kEPW;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int le = (int)(t % kEPW);
+    const int q = (int)((t / kEPW) % 27);
+    const int kz = (int)((t / (27L * kEPW)) % a.ZC);
+    const long tile = t / (27L * kEPW * a.ZC);
+    const TileBox tb = tile_box(a, tile);
+    if (kz >= tb.nzt) continue;
+    const int lx = le % a.TX, ly = le / a.TX;
+    const int ex = tb.x0 + min(lx, tb.nxt - 1), ey = tb.y0 + min(ly, tb.nyt - 1);
+    const long e = ex + (long)a.mx * (ey + (long)a.my * (tb.z0 + kz));
+    et[t] = (T)eta[(long)q * a.nel + e];
+  }
 }
 
 // Write the completed node plane P of the window: owned nodes to y, the
@@ -417,7 +455,7 @@ __global__ void __launch_bounds__(64)
                     T *__restrict__ fbx, T *__restrict__ fby, T *__restrict__ fbz) {
   __shared__ T sh[3][kSlots][kEPW];
   __shared__ T win[3][3 * kWinNodes];
-  const long tile = blockIdx.x;
+  const long tile = xcd_tile(blockIdx.x, a.ntiles);
   const TileBox tb = tile_box(a, tile);
   const int lane = threadIdx.x;
   const int c = lane / kEPW;
@@ -433,10 +471,10 @@ __global__ void __launch_bounds__(64)
 
   for (int kz = 0; kz < tb.nzt; kz++) {
     const long ek = tb.z0 + kz;
-    const long e = ex + (long)a.mx * (ey + (long)a.my * ek);
     const long base = 2L * ex + a.X * (2L * ey) + a.XY * (2 * ek + cp);
     T out[3][9];
-    element_plane_apply<T, kMask>(cp, le, c < 3, e, a.nel, base, a.X, xc, eta, bcn, u, sh, out);
+    element_plane_apply<T, kMask>(cp, le, c < 3, eta + tq_index(tile, a.ZC, kz, 9 * cp, le), kEPW, base,
+                                  a.X, xc, bcn, u, sh, out);
     // Sum into the window.  For a fixed node slot (i,j) no two lanes touch
     // the same window entry, so the nine slots are added one after another
     // with plain read-add-write; the wave executes LDS operations in order
@@ -478,7 +516,7 @@ __global__ void __launch_bounds__(64)
                        T *__restrict__ fbx, T *__restrict__ fby, T *__restrict__ fbz) {
   __shared__ T sh[3][kSlots][kEPW];
   __shared__ T win[3][3 * kWinNodes];
-  const long tile = blockIdx.x;
+  const long tile = xcd_tile(blockIdx.x, a.ntiles);
   const TileBox tb = tile_box(a, tile);
   const int lane = threadIdx.x;
   const int c = lane / kEPW;
@@ -489,13 +527,11 @@ __global__ void __launch_bounds__(64)
   const bool live = writer && (le < a.TX * a.TY) && (lx < tb.nxt) && (ly < tb.nyt);
   const int ex = tb.x0 + min(lx, tb.nxt - 1), ey = tb.y0 + min(ly, tb.nyt - 1);
   const int wX = 2 * tb.nxt + 1;
-  const long ecol = ex + (long)a.mx * ey;       // element (ex, ey, 0)
-  const long ncol = 2L * ex + a.X * (2L * ey);  // its node (0, 0, 0)
-  const long nlay = (long)a.mx * a.my;
+  const long ncol = 2L * ex + a.X * (2L * ey);  // node (0, 0, 0) of element (ex, ey, 0)
 
   PlaneRaw<T> r;
   load_plane_nodes<T, kMask>(r, xc, u, bcn, ncol + a.XY * (2L * tb.z0 + cp), a.X);
-  load_plane_eta<T>(r, eta + (long)(9 * cp) * a.nel + ecol + nlay * tb.z0, a.nel);
+  load_plane_eta<T>(r, eta + tq_index(tile, a.ZC, 0, 9 * cp, le), kEPW);
   for (int t = lane; t < 9 * kWinNodes; t += 64) (&win[0][0])[t] = T(0);
   __syncthreads();
 
@@ -512,7 +548,7 @@ __global__ void __launch_bounds__(64)
     __builtin_amdgcn_sched_barrier(0);
     if (kz + 1 < tb.nzt) {
       load_plane_nodes<T, kMask>(r, xc, u, bcn, ncol + a.XY * (2 * ek + 2 + cp), a.X);
-      load_plane_eta<T>(r, eta + (long)(9 * cp) * a.nel + ecol + nlay * (ek + 1), a.nel);
+      load_plane_eta<T>(r, eta + tq_index(tile, a.ZC, kz + 1, 9 * cp, le), kEPW);
     }
     __builtin_amdgcn_sched_barrier(0);
     // window sums, as in a11_tile_kernel
@@ -688,7 +724,7 @@ __global__ void __launch_bounds__(64)
                        T *__restrict__ fby, T *__restrict__ fbz) {
   __shared__ T sh[3][27][kEPW];
   __shared__ T win[3][3 * kWinNodes];
-  const long tile = blockIdx.x;
+  const long tile = xcd_tile(blockIdx.x, a.ntiles);
   const TileBox tb = tile_box(a, tile);
   const int lane = threadIdx.x;
   const int c = lane / kEPW;
@@ -821,7 +857,7 @@ template <typename T>
 __global__ void __launch_bounds__(64)
     a11_face_kernel(TileArgs a, const T *__restrict__ fbx,
                     const T *__restrict__ fby, const T *__restrict__ fbz, T *__restrict__ y) {
-  const long tile = blockIdx.x;
+  const long tile = xcd_tile(blockIdx.x, a.ntiles);
   const TileBox tb = tile_box(a, tile);
   const int oX = 2 * tb.nxt + (tb.xend ? 1 : 0);
   const int oY = 2 * tb.nyt + (tb.yend ? 1 : 0);
@@ -875,6 +911,7 @@ TileArgs tile_args(const ptmf_op_s *op) {
   a.X = b.X; a.XY = b.X * b.Y; a.nel = b.nel;
   a.TX = op->tiling.TX; a.TY = op->tiling.TY; a.ZC = op->tiling.ZC;
   a.ntx = op->tiling.ntx; a.nty = op->tiling.nty;
+  a.ntiles = op->tiling.ntiles;
   a.fbx_n = (long)(2 * a.TY + 1) * (2 * a.ZC + 1);
   a.fby_n = (long)(2 * a.TX + 1) * (2 * a.ZC + 1);
   a.fbz_n = (long)(2 * a.TX + 1) * (2 * a.TY + 1);
@@ -903,6 +940,39 @@ int launch_a11(const ptmf_op_s *op, const T *u, T *y, bool mask_columns) {
     a11_atomic_kernel<T, false><<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
         b.mx, b.my, b.nel, b.X, b.X * b.Y, d.coords, d.eta, nullptr, u, y);
   PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+// (Re)build the tile-ordered viscosity for precision T when the coefficients
+// or the tiling changed since the last build.
+template <typename T>
+int ensure_eta_tiled(ptmf_op_s *op) {
+  const int ip = sizeof(T) == 8 ? 0 : 1;
+  const Tiling &tl = op->tiling;
+  if (op->et_state[ip] == op->state && op->et_tiling[ip][0] == tl.TX &&
+      op->et_tiling[ip][1] == tl.TY && op->et_tiling[ip][2] == tl.ZC)
+    return PTMF_OK;
+  const long len = tl.ntiles * tl.ZC * 27L * kEPW;
+  if (len > op->et_len) {
+    PTMF_HIP(hipStreamSynchronize(op->stream));
+    (void)hipFree(op->et64);
+    (void)hipFree(op->et32);
+    op->et64 = nullptr;
+    op->et32 = nullptr;
+    op->et_state[0] = op->et_state[1] = ~0UL;
+    op->et_len = len;
+  }
+  T **dst = (T **)(sizeof(T) == 8 ? (void *)&op->et64 : (void *)&op->et32);
+  if (!*dst) PTMF_HIP(hipMalloc((void **)dst, sizeof(T) * op->et_len));
+  const TileArgs a = tile_args(op);
+  long nb = (len + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  eta_tile_kernel<T><<<dim3((unsigned)nb), dim3(256), 0, op->stream>>>(a, op->d64.eta, *dst);
+  PTMF_HIP(hipGetLastError());
+  op->et_state[ip] = op->state;
+  op->et_tiling[ip][0] = tl.TX;
+  op->et_tiling[ip][1] = tl.TY;
+  op->et_tiling[ip][2] = tl.ZC;
   return PTMF_OK;
 }
 
@@ -966,12 +1036,16 @@ int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, bool faces) {
     const T *pa = sizeof(T) == 8 ? (const T *)op->pa64 : (const T *)op->pa32;
     a11_tile_pa_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
         a, pa, op->bcnode, x, y, fbx, fby, fbz);
-  } else if (op->variant == 1)
-    a11_tile_pf_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
-        a, d.coords, d.eta, op->bcnode, x, y, fbx, fby, fbz);
-  else
-    a11_tile_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
-        a, d.coords, d.eta, op->bcnode, x, y, fbx, fby, fbz);
+  } else {
+    PTMF_TRY(ensure_eta_tiled<T>(op));
+    const T *et = sizeof(T) == 8 ? (const T *)op->et64 : (const T *)op->et32;
+    if (op->variant == 1)
+      a11_tile_pf_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
+          a, d.coords, et, op->bcnode, x, y, fbx, fby, fbz);
+    else
+      a11_tile_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
+          a, d.coords, et, op->bcnode, x, y, fbx, fby, fbz);
+  }
   PTMF_HIP(hipGetLastError());
   if (faces) {
     a11_face_kernel<T><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(a, fbx, fby, fbz, y);

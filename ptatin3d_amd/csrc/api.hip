@@ -49,7 +49,7 @@ Tiling choose_tiling(int mx, int my, int mz) {
       }
     }
   const long txy = (long)((mx + bestx - 1) / bestx) * ((my + besty - 1) / besty);
-  long zc = (long)mz * txy / 3072;
+  long zc = (long)mz * txy / 6144;
   if (zc < 2) zc = 2;
   if (zc > 16) zc = 16;
   if (zc > mz) zc = mz;
@@ -184,6 +184,8 @@ int ptmf_op_destroy(ptmf_op *pop) {
   (void)hipFree(op->pa64);
   (void)hipFree(op->pa32);
   (void)hipFree(op->pa_bad);
+  (void)hipFree(op->et64);
+  (void)hipFree(op->et32);
   if (op->ev0) (void)hipEventDestroy(op->ev0);
   if (op->ev1) (void)hipEventDestroy(op->ev1);
   if (op->stream) (void)hipStreamDestroy(op->stream);

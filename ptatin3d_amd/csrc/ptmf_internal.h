@@ -94,6 +94,12 @@ struct ptmf_op_s {
   unsigned long pa_state[2] = {~0UL, ~0UL};
   int pa_tiling[2][3] = {{0, 0, 0}, {0, 0, 0}};
   int *pa_bad = nullptr;
+  // tile-ordered viscosity of variants 0/1 (per precision), same validity keys
+  double *et64 = nullptr;
+  float *et32 = nullptr;
+  long et_len = 0;
+  unsigned long et_state[2] = {~0UL, ~0UL};
+  int et_tiling[2][3] = {{0, 0, 0}, {0, 0, 0}};
   void *faces = nullptr;          // tile face buffers (sized for FP64)
   long faces_doubles = 0;
   double *host_stage = nullptr;   // device staging for the host-array entry points

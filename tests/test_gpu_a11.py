@@ -72,11 +72,13 @@ def test_tilings(gpu, tiling):
     assert rel_inf(y, yo) <= TOL64
     y32 = A.mult_host(x, precision=32)
     assert rel_inf(y32, yo) <= TOL32
-    # the software-pipelined kernel sums in the same order: identical results
+    # the software-pipelined kernel: the same sums, compiled in another
+    # context (FMA contraction may differ), so equal to rounding
     A.set_kernel_variant(1)
     assert A.get_kernel_variant() == 1
-    assert np.array_equal(A.mult_host(x), y)
-    assert np.array_equal(A.mult_host(x, precision=32), y32)
+    assert rel_inf(A.mult_host(x), y) <= 1e-14
+    assert rel_inf(A.mult_host(x), yo) <= TOL64
+    assert rel_inf(A.mult_host(x, precision=32), y32) <= 1e-6
     # stored quadrature-point geometry: the same operator to rounding
     A.set_kernel_variant(2)
     assert rel_inf(A.mult_host(x), yo) <= TOL64
