@@ -75,6 +75,12 @@ typedef struct {
   int bc_transfer;        /* 1: the coarse correction vanishes on Dirichlet dofs (the
                              restricted residual is zeroed there); 0: plain R = P^T as
                              PCMG with DMDA interpolation (ptatin_driver_linear_ts.c:849) */
+  int smoother_diag;      /* 0: point Jacobi (diag(A), -mg_levels_pc_type jacobi); 1: l1-Jacobi
+                             (sum_j |a_ij|, which bounds the spectrum of D^-1 A by 1 where
+                             point Jacobi lets elements cut by viscosity jumps stretch it) */
+  int transfer;           /* 0: DMDA Q1 interpolation on the Q2 node lattice (the reference's
+                             DMCreateInterpolation, ptatin_driver_linear_ts.c:989); 1: Q2
+                             embedding (fine nodal values of the coarse Q2 function) */
 } ptmf_mg_options;
 
 /* ---- library / device helpers ------------------------------------------ */
@@ -135,6 +141,12 @@ int ptmf_a11_mult(ptmf_op op, const double *x, double *y);
 int ptmf_a11_mult_f32(ptmf_op op, const float *x, float *y);
 /* MatGetDiagonal_MFStokes_A11: diag(A11), 1 in Dirichlet slots. */
 int ptmf_a11_diagonal(ptmf_op op, double *d);
+/* l1 row sums of the A11 MatMult operator (device vector 3N): for each
+ * non-Dirichlet row, the sum over elements of the absolute entries of the
+ * element matrix in non-Dirichlet columns (an upper bound of sum_j |a_ij|);
+ * Dirichlet rows 1.  The scaling of l1-Jacobi smoothing; no counterpart in
+ * the reference, whose smoother is point Jacobi (PCJACOBI). */
+int ptmf_a11_l1_diagonal(ptmf_op op, double *d);
 
 /* ---- B / B^T ------------------------------------------------------------- */
 /* MatMult_MFStokes_A12: yu = B p (p: 4E), Dirichlet velocity rows set to 0. */
@@ -167,10 +179,12 @@ int ptmf_mg_level_op(ptmf_mg mg, int level, ptmf_op *op);
 int ptmf_mg_set_coordinates(ptmf_mg mg, const double *coords);
 /* quadrature-point viscosity of one level (host, 27E, element-major) */
 int ptmf_mg_set_viscosity(ptmf_mg mg, int level, const double *eta);
-/* fill every coarser level from the next finer one: each coarse element takes
- * one mean of its 8 children's 216 quadrature values; mode 0 arithmetic,
- * 1 geometric, 2 harmonic.  (The reference projects marker viscosity through a
- * Q1 nodal field per level, material_point_utils.c:1267-1420.) */
+/* fill every coarser level from the next finer one.  Modes 0/1/2: each coarse
+ * element takes one arithmetic/geometric/harmonic mean of its 8 children's 216
+ * quadrature values.  Modes 3/4: each coarse quadrature point takes the
+ * arithmetic/geometric mean of the children containing it (sub-element
+ * variation kept).  (The reference projects marker viscosity through a Q1
+ * nodal field per level, material_point_utils.c:1267-1420.) */
 int ptmf_mg_coarsen_viscosity(ptmf_mg mg, int mode);
 /* Dirichlet mask of one level (host, 3N bytes) */
 int ptmf_mg_set_dirichlet(ptmf_mg mg, int level, const unsigned char *mask);

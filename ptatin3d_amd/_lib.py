@@ -59,6 +59,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_a11_mult": (_I, [_P, _P, _P]),
     "ptmf_a11_mult_f32": (_I, [_P, _P, _P]),
     "ptmf_a11_diagonal": (_I, [_P, _P]),
+    "ptmf_a11_l1_diagonal": (_I, [_P, _P]),
     "ptmf_a12_mult": (_I, [_P, _P, _P]),
     "ptmf_a21_mult": (_I, [_P, _P, _P]),
     "ptmf_a11_time": (_I, [_P, _I, _P, _P, _I, _PD]),
@@ -99,6 +100,8 @@ class MGOptions(C.Structure):
         ("coarse_maxit", C.c_int),
         ("galerkin_levels", C.c_int),
         ("bc_transfer", C.c_int),
+        ("smoother_diag", C.c_int),
+        ("transfer", C.c_int),
     ]
 
 

@@ -392,6 +392,16 @@ int ptmf_a11_diagonal(ptmf_op op, double *d) {
   return PTMF_OK;
 }
 
+int ptmf_a11_l1_diagonal(ptmf_op op, double *d) {
+  PTMF_TRY(check_ready(op, true, true));
+  PTMF_CHECK(d, PTMF_ERR_ARG, "a11_l1_diagonal: null");
+  PTMF_TRY(ptmf::launch_fill<double>(d, 0.0, 3 * op->box.nnodes, op->stream));
+  PTMF_TRY(ptmf::launch_a11_l1(op, d));
+  PTMF_TRY(ptmf::launch_bc_set<double>(op->bcidx, op->nbc, 1.0, d, op->stream));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
 int ptmf_a12_mult(ptmf_op op, const double *p, double *yu) {
   PTMF_TRY(check_ready(op, false, true));
   PTMF_CHECK(p && yu, PTMF_ERR_ARG, "a12_mult: null");

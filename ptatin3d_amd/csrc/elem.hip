@@ -131,6 +131,81 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// ---- l1 row sums of A11 with the MatMult's boundary handling, an upper
+// bound of sum_j |a_ij| formed element by element:
+//   l1[3n+c] += sum_{(m,e) not Dirichlet} |A_el[(n,c),(m,e)]|,
+//   A_el[(n,c),(m,e)] = sum_q w detJ eta (delta_ce grad N_n . grad N_m + d_e N_n d_c N_m)
+// (the element matrix of stokes_assembly.c:219-268).  Dirichlet rows are set
+// to 1 afterwards (identity rows).  One element per 128-lane work-group;
+// lane t < 81 owns element row t = 3 a + c.  Setup-time work.
+__global__ void __launch_bounds__(128)
+    a11_l1_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
+                  const double *__restrict__ eta, const uint8_t *__restrict__ bcn,
+                  double *__restrict__ l1) {
+  __shared__ double sx[27][3];
+  __shared__ double sG[27][10];
+  __shared__ double sgp[27][27][3];  // [q][node][k]: physical gradient of N_node at qp q
+  __shared__ unsigned sm[27];
+  const long e = blockIdx.x;
+  const int t = threadIdx.x;
+  if (t < 27) {
+    const long node = elem_node(mx, my, X, XY, e, t);
+    for (int d = 0; d < 3; d++) sx[t][d] = xc[3 * node + d];
+    sm[t] = bcn ? (unsigned)bcn[node] : 0u;
+  }
+  __syncthreads();
+  if (t < 27) {
+    QpGeom g;
+    qp_geometry(sx, t, g);
+    for (int r = 0; r < 3; r++)
+      for (int k = 0; k < 3; k++) sG[t][3 * r + k] = g.G[r][k];
+    sG[t][9] = g.wdet * eta[(long)t * nel + e];
+  }
+  __syncthreads();
+  for (int idx = t; idx < 729; idx += 128) {
+    const int q = idx / 27, n = idx - 27 * q;
+    double gr[3];
+    grad_ref(n, q, gr);
+    const double *G = sG[q];
+    for (int k = 0; k < 3; k++) sgp[q][n][k] = G[3 * k] * gr[0] + G[3 * k + 1] * gr[1] + G[3 * k + 2] * gr[2];
+  }
+  __syncthreads();
+  if (t < 81) {
+    const int a = t / 3, c = t - 3 * a;
+    double acc[27][3];
+#pragma unroll
+    for (int b = 0; b < 27; b++) acc[b][0] = acc[b][1] = acc[b][2] = 0.0;
+    for (int q = 0; q < 27; q++) {
+      const double s = sG[q][9];
+      const double ga0 = sgp[q][a][0], ga1 = sgp[q][a][1], ga2 = sgp[q][a][2];
+#pragma unroll
+      for (int b = 0; b < 27; b++) {
+        const double gb0 = sgp[q][b][0], gb1 = sgp[q][b][1], gb2 = sgp[q][b][2];
+        const double gbc = c == 0 ? gb0 : (c == 1 ? gb1 : gb2);
+        const double dot = s * (ga0 * gb0 + ga1 * gb1 + ga2 * gb2);
+        const double sgbc = s * gbc;
+        acc[b][0] += ga0 * sgbc;
+        acc[b][1] += ga1 * sgbc;
+        acc[b][2] += ga2 * sgbc;
+        // delta_ce term; the branches keep the register indices static
+        if (c == 0) acc[b][0] += dot;
+        else if (c == 1) acc[b][1] += dot;
+        else acc[b][2] += dot;
+      }
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int b = 0; b < 27; b++) {
+      const unsigned mb = sm[b];
+#pragma unroll
+      for (int ee = 0; ee < 3; ee++)
+        if (!((mb >> ee) & 1u)) sum += fabs(acc[b][ee]);
+    }
+    const long node = elem_node(mx, my, X, XY, e, a);
+    atomicAdd(&l1[3 * node + c], sum);
+  }
+}
+
 // ---- B p: yu[3n+c] += sum_q -w detJ (d_c N_n) p_h(x_q)
 // (MFStokesWrapper_A12, stokes_operators_mf.c:788-856 with
 // MatMultMF_Stokes_MixedFEM3d_A12, stokes_q2p1_mf_operators_def.c:423-516)
@@ -239,6 +314,16 @@ int launch_a11_diag(const ptmf_op_s *op, double *d) {
   const long nblk = (b.nel + kEPB - 1) / kEPB;
   a11_diag_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
       b.mx, b.my, b.nel, b.X, b.X * b.Y, op->d64.coords, op->d64.eta, d);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+int launch_a11_l1(const ptmf_op_s *op, double *d) {
+  PTMF_TRY(init_constants());
+  const Box &b = op->box;
+  PTMF_CHECK(b.nel < (1L << 31), PTMF_ERR_ARG, "a11_l1: mesh too large");
+  a11_l1_kernel<<<dim3((unsigned)b.nel), dim3(128), 0, op->stream>>>(
+      b.mx, b.my, b.nel, b.X, b.X * b.Y, op->d64.coords, op->d64.eta, op->bcnode, d);
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }

@@ -254,6 +254,96 @@ __global__ void k_restrict(long Xc, long Yc, long Zc, long Xf, long Yf, long Zf,
   }
 }
 
+// Q2 embedding (transfer option 1): the fine nodal values of the coarse Q2
+// function.  Per direction, fine node f = 2c coincides with coarse node c;
+// an odd fine node between c and c+1 lies at xi = -1/2 (c even) or +1/2
+// (c odd) of the coarse element and takes the quadratic Lagrange weights
+// (3/8, 3/4, -1/8) from the element's nodes.
+__device__ __forceinline__ int q2_weights(long f, long *c, double *w) {
+  if (!(f & 1)) { c[0] = f >> 1; w[0] = 1.0; return 1; }
+  const long cl = f >> 1;  // coarse node left of f
+  if (!(cl & 1)) {         // xi = -1/2 in element [cl, cl+2]
+    c[0] = cl; c[1] = cl + 1; c[2] = cl + 2;
+    w[0] = 0.375; w[1] = 0.75; w[2] = -0.125;
+  } else {                 // xi = +1/2 in element [cl-1, cl+1]
+    c[0] = cl - 1; c[1] = cl; c[2] = cl + 1;
+    w[0] = -0.125; w[1] = 0.75; w[2] = 0.375;
+  }
+  return 3;
+}
+
+template <typename T>
+__global__ void k_prolong_add_q2(long Xc, long Yc, long Xf, long Yf, long Zf, const T *__restrict__ xc,
+                                 T *__restrict__ yf) {
+  const long nf = Xf * Yf * Zf;
+  for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < nf; f += (long)gridDim.x * blockDim.x) {
+    const long I = f % Xf, J = (f / Xf) % Yf, K = f / (Xf * Yf);
+    long ci[3], cj[3], ck[3];
+    double wi[3], wj[3], wk[3];
+    const int ni = q2_weights(I, ci, wi), nj = q2_weights(J, cj, wj), nk = q2_weights(K, ck, wk);
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int c = 0; c < nk; c++)
+      for (int b = 0; b < nj; b++)
+        for (int a = 0; a < ni; a++) {
+          const long g = ci[a] + Xc * (cj[b] + Yc * ck[c]);
+          const T w = (T)(wi[a] * wj[b] * wk[c]);
+          s0 += w * xc[3 * g];
+          s1 += w * xc[3 * g + 1];
+          s2 += w * xc[3 * g + 2];
+        }
+    yf[3 * f] += s0;
+    yf[3 * f + 1] += s1;
+    yf[3 * f + 2] += s2;
+  }
+}
+
+// transposed Q2 embedding, gathered per coarse node: per direction the fine
+// offsets from 2c with weights {-3: -1/8, -1: 3/8, 0: 1, 1: 3/8, 3: -1/8}
+// (c even) or {-1: 3/4, 0: 1, 1: 3/4} (c odd)
+__device__ __forceinline__ int q2_rweights(long c, int *o, double *w) {
+  if (c & 1) {
+    o[0] = -1; o[1] = 0; o[2] = 1;
+    w[0] = 0.75; w[1] = 1.0; w[2] = 0.75;
+    return 3;
+  }
+  o[0] = -3; o[1] = -1; o[2] = 0; o[3] = 1; o[4] = 3;
+  w[0] = -0.125; w[1] = 0.375; w[2] = 1.0; w[3] = 0.375; w[4] = -0.125;
+  return 5;
+}
+
+template <typename T>
+__global__ void k_restrict_q2(long Xc, long Yc, long Zc, long Xf, long Yf, long Zf, const T *__restrict__ xf,
+                              T *__restrict__ yc) {
+  const long ncn = Xc * Yc * Zc;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < ncn; g += (long)gridDim.x * blockDim.x) {
+    const long i = g % Xc, j = (g / Xc) % Yc, k = g / (Xc * Yc);
+    int oi[5], oj[5], ok[5];
+    double wi[5], wj[5], wk[5];
+    const int ni = q2_rweights(i, oi, wi), nj = q2_rweights(j, oj, wj), nk = q2_rweights(k, ok, wk);
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int c = 0; c < nk; c++) {
+      const long K = 2 * k + ok[c];
+      if (K < 0 || K >= Zf) continue;
+      for (int b = 0; b < nj; b++) {
+        const long J = 2 * j + oj[b];
+        if (J < 0 || J >= Yf) continue;
+        for (int a = 0; a < ni; a++) {
+          const long I = 2 * i + oi[a];
+          if (I < 0 || I >= Xf) continue;
+          const T w = (T)(wi[a] * wj[b] * wk[c]);
+          const long f = I + Xf * (J + Yf * K);
+          s0 += w * xf[3 * f];
+          s1 += w * xf[3 * f + 1];
+          s2 += w * xf[3 * f + 2];
+        }
+      }
+    }
+    yc[3 * g] = s0;
+    yc[3 * g + 1] = s1;
+    yc[3 * g + 2] = s2;
+  }
+}
+
 // coarse-element viscosity from the 8 x 27 quadrature values of the children
 // (element-major input and output): mode 0 arithmetic, 1 geometric,
 // 2 harmonic mean, one value per coarse element
@@ -276,6 +366,36 @@ __global__ void k_coarsen_eta(int mxc, int myc, int mzc, int mode, const double 
     s *= 1.0 / 216.0;
     const double m = mode == 0 ? s : (mode == 1 ? exp(s) : 1.0 / s);
     for (int q = 0; q < 27; q++) ec[27 * E + q] = m;
+  }
+}
+
+// quadrature-point-local coarsening (modes 3 arithmetic, 4 geometric): coarse
+// quadrature point (qa, qb, qc) averages the 27 fine values of each child
+// element containing it (per direction: child 0 for qa = 0, child 1 for
+// qa = 2, both for the midpoint qa = 1), keeping the sub-element variation
+__global__ void k_coarsen_eta_qp(int mxc, int myc, int mzc, int mode, const double *__restrict__ ef,
+                                 double *__restrict__ ec) {
+  const long n = 27L * mxc * myc * mzc;
+  const int mxf = 2 * mxc, myf = 2 * myc;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const long E = t / 27;
+    const int q = (int)(t - 27 * E);
+    const int qa = q % 3, qb = (q / 3) % 3, qc = q / 9;
+    const long ei = E % mxc, ej = (E / mxc) % myc, ek = E / ((long)mxc * myc);
+    double s = 0.0;
+    int cnt = 0;
+    for (int c = (qc == 2 ? 1 : 0); c <= (qc == 0 ? 0 : 1); c++)
+      for (int b = (qb == 2 ? 1 : 0); b <= (qb == 0 ? 0 : 1); b++)
+        for (int a = (qa == 2 ? 1 : 0); a <= (qa == 0 ? 0 : 1); a++) {
+          const long ef_e = (2 * ei + a) + (long)mxf * ((2 * ej + b) + (long)myf * (2 * ek + c));
+          for (int r = 0; r < 27; r++) {
+            const double v = ef[27 * ef_e + r];
+            s += mode == 3 ? v : log(v);
+          }
+          cnt += 27;
+        }
+    s /= (double)cnt;
+    ec[t] = mode == 3 ? s : exp(s);
   }
 }
 
@@ -358,6 +478,17 @@ __global__ void k_stencil_bc_identity(long N, const long *__restrict__ idx, long
     const int d = (int)(g - 3 * m);
     S[((long)oc * 9 + 4 * d) * N + m] = 1.0;
   }
+}
+
+// l1 row sums of a Galerkin stencil: sum over offsets and columns of |S|
+__global__ void k_stencil_l1(long N, const double *__restrict__ S, double *__restrict__ dg) {
+  for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < N; m += (long)gridDim.x * blockDim.x)
+    for (int r = 0; r < 3; r++) {
+      double s = 0.0;
+      for (int o = 0; o < 125; o++)
+        for (int c = 0; c < 3; c++) s += fabs(S[((long)o * 9 + 3 * r + c) * N + m]);
+      dg[3 * m + r] = s;
+    }
 }
 
 __global__ void k_stencil_diag(long N, const double *__restrict__ S, double *__restrict__ dg) {
@@ -477,6 +608,30 @@ int matmult(const MGLevel &L, const T *x, T *y) {
   return PTMF_OK;
 }
 
+// transfers between level l (fine) and l-1: DMDA Q1 lattice interpolation
+// (transfer 0, as the reference's PCMG) or the Q2 embedding (transfer 1)
+template <typename T>
+int prolong_add(ptmf_mg_s *mg, int l, const T *xc, T *yf) {
+  const Box &bf = mg->lev[l].op->box, &bc = mg->lev[l - 1].op->box;
+  if (mg->opt.transfer == 1)
+    k_prolong_add_q2<T><<<nblocks(bf.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bf.X, bf.Y, bf.Z, xc, yf);
+  else
+    k_prolong_add<T><<<nblocks(bf.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bf.X, bf.Y, bf.Z, xc, yf);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+template <typename T>
+int restrict_(ptmf_mg_s *mg, int l, const T *xf, T *yc) {
+  const Box &bf = mg->lev[l].op->box, &bc = mg->lev[l - 1].op->box;
+  if (mg->opt.transfer == 1)
+    k_restrict_q2<T><<<nblocks(bc.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z, xf, yc);
+  else
+    k_restrict<T><<<nblocks(bc.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z, xf, yc);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
 // Chebyshev-Jacobi smoothing on level L (precision T): x <- cheb(b, x)
 template <typename T>
 int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
@@ -561,15 +716,13 @@ int vcycle(ptmf_mg_s *mg, int l, int lc, const T *b, T *x) {
   k_residual<T><<<nblocks(L.n), kBS, 0, s>>>(L.n, b, r, r);
   PTMF_HIP(hipGetLastError());
   const Box &bf = L.op->box, &bcx = C.op->box;
-  k_restrict<T><<<nblocks(bcx.nnodes), kBS, 0, s>>>(bcx.X, bcx.Y, bcx.Z, bf.X, bf.Y, bf.Z, r, bc);
-  PTMF_HIP(hipGetLastError());
+  PTMF_TRY(restrict_<T>(mg, l, r, bc));
   // coarse correction vanishing on the Dirichlet dofs: the coarse Dirichlet
   // rows are identities, so a nonzero restricted residual there would act as
   // boundary data of the correction
   if (mg->opt.bc_transfer) PTMF_TRY(launch_bc_set<T>(C.op->bcidx, C.op->nbc, (T)0, bc, s));
   PTMF_TRY(vcycle<T>(mg, l - 1, lc, bc, xc));
-  k_prolong_add<T><<<nblocks(bf.nnodes), kBS, 0, s>>>(bcx.X, bcx.Y, bf.X, bf.Y, bf.Z, xc, x);
-  PTMF_HIP(hipGetLastError());
+  PTMF_TRY(prolong_add<T>(mg, l, xc, x));
   // post-smooth from the corrected iterate
   PTMF_TRY(cheb<T>(mg, L, b, x, mg->opt.post_its, true));
   return PTMF_OK;
@@ -657,6 +810,9 @@ int estimate_emax(ptmf_mg_s *mg, int l, int its, double *emax) {
     h[i] = ((double)(st >> 11) / (double)(1UL << 53)) - 0.5;
   }
   PTMF_HIP(hipMemcpyAsync(r, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  // l1 scaling: estimate on the interior block (the Dirichlet block, identity
+  // rows over masked columns, is decoupled and gets its own scaling)
+  if (mg->opt.smoother_diag == 1) PTMF_TRY(launch_bc_set<double>(L.op->bcidx, L.op->nbc, 0.0, r, s));
   PTMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
   std::vector<double> alpha, beta;
   // z = D^-1 r
@@ -695,6 +851,18 @@ int estimate_emax(ptmf_mg_s *mg, int l, int its, double *emax) {
   return PTMF_OK;
 }
 
+// With l1 scaling the decoupled Dirichlet dofs (identity rows) would sit at
+// eigenvalue 1 of D^-1 A, outside the interior spectrum: scale them to the
+// middle of the Chebyshev interval instead.  Point Jacobi keeps unit
+// Dirichlet diagonals, as PCJACOBI does.
+int place_dirichlet(ptmf_mg_s *mg, MGLevel &L) {
+  if (mg->opt.smoother_diag != 1) return PTMF_OK;
+  const double mid = 0.5 * (L.emin + L.emax);
+  PTMF_TRY(launch_bc_set<double>(L.op->bcidx, L.op->nbc, mid, L.dinv64, mg->stream));
+  PTMF_TRY(launch_bc_set<float>(L.op->bcidx, L.op->nbc, (float)mid, L.dinv32, mg->stream));
+  return PTMF_OK;
+}
+
 // Galerkin operator of level l: S_l = R A_{l+1} P by probing with the 375
 // colourings (node indices modulo 5, one component each); the coupling range
 // of R A P stays within +-2 nodes, so every probe yields one stencil entry
@@ -717,9 +885,9 @@ int assemble_galerkin(ptmf_mg_s *mg, int l) {
           k_probe<<<nblocks(N), kBS, 0, s>>>(bc.X, bc.Y, bc.Z, ci, cj, ck, d, v);
           PTMF_TRY(launch_bc_set<double>(Cl.op->bcidx, Cl.op->nbc, 0.0, v, s));
           PTMF_HIP(hipMemsetAsync(pf, 0, sizeof(double) * Fl.n, s));
-          k_prolong_add<double><<<nblocks(bf.nnodes), kBS, 0, s>>>(bc.X, bc.Y, bf.X, bf.Y, bf.Z, v, pf);
+          PTMF_TRY(prolong_add<double>(mg, l + 1, v, pf));
           PTMF_TRY(matmult<double>(Fl, pf, apf));
-          k_restrict<double><<<nblocks(N), kBS, 0, s>>>(bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z, apf, w);
+          PTMF_TRY(restrict_<double>(mg, l + 1, apf, w));
           PTMF_TRY(launch_bc_set<double>(Cl.op->bcidx, Cl.op->nbc, 0.0, w, s));
           k_probe_scatter<<<nblocks(N), kBS, 0, s>>>(bc.X, bc.Y, bc.Z, ci, cj, ck, d, w, Cl.S64);
           PTMF_HIP(hipGetLastError());
@@ -786,6 +954,8 @@ int ptmf_mg_default_options(ptmf_mg_options *o) {
   o->coarse_maxit = 200;
   o->galerkin_levels = 0;
   o->bc_transfer = 1;
+  o->transfer = 0;
+  o->smoother_diag = 0;
   return PTMF_OK;
 }
 
@@ -927,7 +1097,9 @@ int ptmf_mg_set_viscosity(ptmf_mg mg, int level, const double *eta) {
 
 int ptmf_mg_coarsen_viscosity(ptmf_mg mg, int mode) {
   PTMF_CHECK(mg, PTMF_ERR_ARG, "mg_coarsen_viscosity: null");
-  PTMF_CHECK(mode >= 0 && mode <= 2, PTMF_ERR_UNSUPPORTED, "mg_coarsen_viscosity: mode %d (0 arithmetic, 1 geometric, 2 harmonic)", mode);
+  PTMF_CHECK(mode >= 0 && mode <= 4, PTMF_ERR_UNSUPPORTED,
+             "mg_coarsen_viscosity: mode %d (0 arithmetic, 1 geometric, 2 harmonic element means; "
+             "3 arithmetic, 4 geometric quadrature-point-local means)", mode);
   PTMF_CHECK(mg->lev[mg->L - 1].have_eta, PTMF_ERR_NOT_SETUP, "mg_coarsen_viscosity: finest viscosity not set");
   for (int l = mg->L - 1; l > 0; l--) {
     MGLevel &F = mg->lev[l], &Cl = mg->lev[l - 1];
@@ -941,7 +1113,11 @@ int ptmf_mg_coarsen_viscosity(ptmf_mg mg, int mode) {
     for (long e = 0; e < nef; e++)
       for (int q = 0; q < 27; q++) he[27 * e + q] = hq[(long)q * nef + e];
     PTMF_HIP(hipMemcpy(ef, he.data(), sizeof(double) * 27 * nef, hipMemcpyHostToDevice));
-    ptmf::k_coarsen_eta<<<ptmf::nblocks(nec), ptmf::kBS, 0, mg->stream>>>(Cl.mx, Cl.my, Cl.mz, mode, ef, ec);
+    if (mode <= 2)
+      ptmf::k_coarsen_eta<<<ptmf::nblocks(nec), ptmf::kBS, 0, mg->stream>>>(Cl.mx, Cl.my, Cl.mz, mode, ef, ec);
+    else
+      ptmf::k_coarsen_eta_qp<<<ptmf::nblocks(27 * nec), ptmf::kBS, 0, mg->stream>>>(Cl.mx, Cl.my, Cl.mz, mode, ef,
+                                                                                  ec);
     PTMF_HIP(hipGetLastError());
     std::vector<double> hc(27 * nec);
     PTMF_HIP(hipMemcpyAsync(hc.data(), ec, sizeof(double) * 27 * nec, hipMemcpyDeviceToHost, mg->stream));
@@ -971,6 +1147,10 @@ int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt) {
   PTMF_CHECK(o.smoother_precision == 32 || o.smoother_precision == 64, PTMF_ERR_UNSUPPORTED,
              "mg_setup: smoother precision %d (32 or 64)", o.smoother_precision);
   PTMF_CHECK(o.pre_its >= 0 && o.post_its >= 0 && o.esteig_its > 0, PTMF_ERR_ARG, "mg_setup: bad iteration counts");
+  PTMF_CHECK(o.transfer == 0 || o.transfer == 1, PTMF_ERR_UNSUPPORTED, "mg_setup: transfer %d (0 or 1)",
+             o.transfer);
+  PTMF_CHECK(o.smoother_diag == 0 || o.smoother_diag == 1, PTMF_ERR_UNSUPPORTED,
+             "mg_setup: smoother_diag %d (0 Jacobi, 1 l1-Jacobi)", o.smoother_diag);
   PTMF_CHECK(o.galerkin_levels >= 0 && o.galerkin_levels < mg->L, PTMF_ERR_ARG,
              "mg_setup: galerkin_levels %d (0..nlevels-1)", o.galerkin_levels);
   hipStream_t s = mg->stream;
@@ -987,8 +1167,13 @@ int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt) {
     PTMF_CHECK(L.op->has_coords && (L.have_eta || L.galerkin) && L.have_bc, PTMF_ERR_NOT_SETUP,
                "mg_setup: level %d lacks coordinates, viscosity or Dirichlet list", l);
     if (L.galerkin) {
-      ptmf::k_stencil_diag<<<ptmf::nblocks(L.op->box.nnodes), ptmf::kBS, 0, s>>>(L.op->box.nnodes, L.S64, L.diag);
+      if (o.smoother_diag == 1)
+        ptmf::k_stencil_l1<<<ptmf::nblocks(L.op->box.nnodes), ptmf::kBS, 0, s>>>(L.op->box.nnodes, L.S64, L.diag);
+      else
+        ptmf::k_stencil_diag<<<ptmf::nblocks(L.op->box.nnodes), ptmf::kBS, 0, s>>>(L.op->box.nnodes, L.S64, L.diag);
       PTMF_HIP(hipGetLastError());
+    } else if (o.smoother_diag == 1) {
+      PTMF_TRY(ptmf_a11_l1_diagonal(L.op, L.diag));
     } else {
       PTMF_TRY(ptmf_a11_diagonal(L.op, L.diag));
     }
@@ -1002,6 +1187,7 @@ int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt) {
       // with the default transform only the max estimate enters
       L.emin = o.esteig[0] * est + o.esteig[1] * est;
       L.emax = o.esteig[2] * est + o.esteig[3] * est;
+      PTMF_TRY(ptmf::place_dirichlet(mg, L));
     }
   }
   PTMF_TRY(ptmf::assemble_dense(mg));
@@ -1031,6 +1217,7 @@ int ptmf_mg_set_level_eigs(ptmf_mg mg, int level, double emin, double emax) {
              "mg_set_level_eigs: bad arguments");
   mg->lev[level].emin = emin;
   mg->lev[level].emax = emax;
+  if (mg->is_setup) PTMF_TRY(ptmf::place_dirichlet(mg, mg->lev[level]));
   return PTMF_OK;
 }
 
@@ -1065,18 +1252,14 @@ int ptmf_mg_level_apply(ptmf_mg mg, int level, int precision, const void *x, voi
 
 int ptmf_mg_prolong_add(ptmf_mg mg, int level, const double *xc, double *yf) {
   PTMF_CHECK(mg && xc && yf && level >= 1 && level < mg->L, PTMF_ERR_ARG, "mg_prolong_add: bad level %d", level);
-  const ptmf::Box &bf = mg->lev[level].op->box, &bc = mg->lev[level - 1].op->box;
-  ptmf::k_prolong_add<double><<<ptmf::nblocks(bf.nnodes), ptmf::kBS, 0, mg->stream>>>(bc.X, bc.Y, bf.X, bf.Y, bf.Z, xc, yf);
-  PTMF_HIP(hipGetLastError());
+  PTMF_TRY(ptmf::prolong_add<double>(mg, level, xc, yf));
   PTMF_HIP(hipStreamSynchronize(mg->stream));
   return PTMF_OK;
 }
 
 int ptmf_mg_restrict(ptmf_mg mg, int level, const double *xf, double *yc) {
   PTMF_CHECK(mg && xf && yc && level >= 1 && level < mg->L, PTMF_ERR_ARG, "mg_restrict: bad level %d", level);
-  const ptmf::Box &bf = mg->lev[level].op->box, &bc = mg->lev[level - 1].op->box;
-  ptmf::k_restrict<double><<<ptmf::nblocks(bc.nnodes), ptmf::kBS, 0, mg->stream>>>(bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z, xf, yc);
-  PTMF_HIP(hipGetLastError());
+  PTMF_TRY(ptmf::restrict_<double>(mg, level, xf, yc));
   PTMF_HIP(hipStreamSynchronize(mg->stream));
   return PTMF_OK;
 }

@@ -97,6 +97,10 @@ class MatA11MF:
         """MatGetDiagonal_MFStokes_A11 (stokes_operators.c:1060-1099)."""
         call("ptmf_a11_diagonal", self.handle, d.ptr)
 
+    def get_l1_diagonal(self, d: DeviceArray) -> None:
+        """l1 row sums of the MatMult operator (the l1-Jacobi scaling)."""
+        call("ptmf_a11_l1_diagonal", self.handle, d.ptr)
+
     # -- host conveniences --------------------------------------------------
     def mult_host(self, x: np.ndarray, precision: int = 64) -> np.ndarray:
         dt = np.float64 if precision == 64 else np.float32
@@ -113,6 +117,13 @@ class MatA11MF:
         u = np.ascontiguousarray(u, dtype=np.float64)
         assert Yu.dtype == np.float64 and Yu.flags.c_contiguous
         call("ptmf_a11_spmv_host", self.handle, _lib._hp(u), _lib._hp(Yu))
+
+    def l1_diagonal_host(self) -> np.ndarray:
+        d = DeviceArray(3 * self.nnodes)
+        self.get_l1_diagonal(d)
+        out = d.download()
+        d.free()
+        return out
 
     def diagonal_host(self) -> np.ndarray:
         d = DeviceArray(3 * self.nnodes)

@@ -140,6 +140,34 @@ def test_diagonal_parity(gpu):
         A.destroy()
 
 
+def test_l1_diagonal_against_assembled_element_matrices(gpu):
+    """l1 row sums = sum over elements of |element matrix| entries in
+    non-Dirichlet columns (oracle element matrices, stokes_assembly.c:219-268),
+    identity on Dirichlet rows; they bound the spectrum of D^-1 A by 1."""
+    m = (3, 2, 4)
+    coords, eta, bc, _ = _case(m, perturb=0.1, lo=1e-3, hi=1e3, seed=21)
+    A = _op(m, coords, eta, bc)
+    d = A.l1_diagonal_host()
+    nid = F.element_node_ids(*m)
+    keep = ~bc.astype(bool)
+    ref = np.zeros(3 * F.nnodes(*m))
+    for e in range(m[0] * m[1] * m[2]):
+        Ae = O.a11_element_matrix(*m, coords, eta, e)
+        dofs = (3 * nid[e][:, None] + np.arange(3)[None, :]).reshape(-1)
+        ref[dofs] += (np.abs(Ae) * keep[dofs][None, :]).sum(axis=1)
+    ref[bc.astype(bool)] = 1.0
+    assert rel_inf(d, ref) <= 1e-13
+    n = ref.size
+    K = np.zeros((n, n))
+    for i in range(n):
+        ei = np.zeros(n)
+        ei[i] = 1.0
+        K[:, i] = O.a11_matmult(*m, coords, eta, bc, ei)
+    lam = np.linalg.eigvalsh(K / np.sqrt(np.outer(d, d)))
+    assert lam[-1] <= 1.0 + 1e-12
+    A.destroy()
+
+
 def test_a12_a21_parity(gpu):
     m = (5, 6, 4)
     coords, eta, bc, u = _case(m)

@@ -166,8 +166,8 @@ def test_galerkin_stencil_is_RAP(gpu):
     mg.destroy()
 
 
-@pytest.mark.parametrize("prec", [64, 32])
-def test_fgmres_galerkin_high_contrast(gpu, prec):
+@pytest.mark.parametrize("prec,sdiag", [(64, 0), (32, 0)])
+def test_fgmres_galerkin_high_contrast(gpu, prec, sdiag):
     """Layered viscosity with weak inclusions (contrast 1e6) on 16^3 with two
     Galerkin coarse levels converges to 1e-8 (FP64 true residual)."""
     m = (16, 16, 16)
@@ -178,10 +178,41 @@ def test_fgmres_galerkin_high_contrast(gpu, prec):
     mg.set_viscosity(3, eta)
     mg.coarsen_viscosity()
     mg.set_dirichlet_all_faces()
-    mg.setup(default_options(smoother_precision=prec, galerkin_levels=2))
+    mg.setup(default_options(smoother_precision=prec, galerkin_levels=2, smoother_diag=sdiag))
     bc = F.dirichlet_all_faces(*m)
     b = F.smooth_body_force(coords, *m)
     x, its, rn, hist, secs = solve_timed(mg, b, rtol=1e-8, maxit=300, restart=30)
     r = b - O.a11_matmult(*m, coords, eta, bc, x)
     assert np.linalg.norm(r) <= 1.5e-8 * np.linalg.norm(b), (its, np.linalg.norm(r) / np.linalg.norm(b))
+    mg.destroy()
+
+
+def test_q2_transfer_reproduces_quadratics_and_is_adjoint(gpu):
+    """The Q2 embedding (transfer 1) interpolates a coarse quadratic field
+    exactly at the fine nodes, and its restriction is the exact transpose."""
+    m = (4, 6, 2)
+    mg, coords, _ = _hierarchy(m, 2, perturb=0.0, opt=default_options(transfer=1))
+    mc = (2, 3, 1)
+    cc = mg.level_coordinates(coords)
+    xf_, xc_ = cc[1].reshape(-1, 3), cc[0].reshape(-1, 3)
+
+    def quad(p):
+        x, y, z = p[:, 0], p[:, 1], p[:, 2]
+        return np.stack([x * x + 2 * x * y - z * z + 3 * y, y * y * z - x + 0.5, x * z + y * y - 2 * z], axis=1).reshape(-1)
+
+    dxc = DeviceArray.from_host(quad(xc_))
+    dyf = DeviceArray(3 * F.nnodes(*m))
+    dyf.zero()
+    mg.prolong_add(1, dxc, dyf)
+    assert rel_inf(dyf.download(), quad(xf_)) <= 1e-14
+    rng = np.random.default_rng(9)
+    a = rng.uniform(-1, 1, 3 * F.nnodes(*mc))
+    b = rng.uniform(-1, 1, 3 * F.nnodes(*m))
+    pa = DeviceArray(b.size)
+    pa.zero()
+    mg.prolong_add(1, DeviceArray.from_host(a), pa)
+    rb = DeviceArray(a.size)
+    mg.restrict(1, DeviceArray.from_host(b), rb)
+    lhs, rhs = pa.download() @ b, a @ rb.download()
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
     mg.destroy()
