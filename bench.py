@@ -7,14 +7,15 @@ Dirichlet rows on all faces.  One step = one MatMult with vectors resident in
 HBM (column masking, element apply, Dirichlet rows).  Prints one JSON line.
 
 The MatMult tiling and kernel variant are picked by the autotuner before the
-timed region.  --solve adds the mixed-precision multigrid solve of the
-layered 1e6-contrast workload (FP32 V-cycle vs FP64 V-cycle, FGMRES in FP64,
-both to a 1e-8 relative FP64 residual) as a "solve" object.  --copy-probe
-runs one plain device copy of 256 MiB after the timed region (the reference
-kernel for HBM byte counters).
+timed region.  The "solve" object is the mixed-precision multigrid solve of
+the layered 1e6-contrast workload (FP32 V-cycle vs FP64 V-cycle, flexible CG
+in FP64, both to a 1e-8 relative FP64 residual), at --solve-m elements per
+direction (default 64; --no-solve skips it).  --copy-probe runs one plain
+device copy of 256 MiB after the timed region (the reference kernel for HBM
+byte counters).
 
   python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--m 64] [--no-cpu]
-                  [--solve [--solve-m 128]] [--copy-probe]
+                  [--solve-m 64 | --no-solve] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -55,7 +56,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiling")
-    ap.add_argument("--solve", action="store_true", help="add the mixed-precision GMG solve leg")
+    ap.add_argument("--no-solve", action="store_true", help="skip the mixed-precision GMG solve leg")
     ap.add_argument("--solve-m", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
@@ -189,7 +190,7 @@ def main() -> int:
         dst.free()
 
     A.destroy()
-    if args.solve:
+    if not args.no_solve:
         out["solve"] = solve_leg(args.solve_m)
     print(json.dumps(out), flush=True)
     return 0
