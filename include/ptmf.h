@@ -153,6 +153,10 @@ int ptmf_a11_l1_diagonal(ptmf_op op, double *d);
 int ptmf_a12_mult(ptmf_op op, const double *p, double *yu);
 /* MatMult_MFStokes_A21: yp = B^T mask(u) (yp: 4E). */
 int ptmf_a21_mult(ptmf_op op, const double *u, double *yp);
+/* Me[16e + 4i + j] = -sum_q w detJ / eta_q NIp_i NIp_j: the element blocks of
+ * the scaled viscous pressure mass matrix used as Schur complement
+ * approximation (MatAssemble_StokesPC_ScaledMassMatrix, stokes_assembly.c:299-450) */
+int ptmf_schur_mass_blocks(ptmf_op op, double *Me);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Enqueues `reps` back-to-back ptmf_a11_mult (precision 64 or 32) on device
@@ -214,6 +218,19 @@ int ptmf_mg_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxi
  * Same stopping rule and outputs as ptmf_mg_fgmres. */
 int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int *its,
                 double *rnorm, double *history, int history_len);
+
+/* ---- saddle point [A11 A12; A21 0] on the finest level ------------------ *
+ * Vectors are [u (3N) | p (4E)] device arrays.  The solver is FGMRES with the
+ * upper block-triangular preconditioner [A11 A12; 0 S]^-1, S the scaled
+ * viscous pressure mass (block diagonal, inverted per element) and A11^-1 one
+ * V-cycle of this multigrid (precision per ptmf_mg_options) -- the
+ * fieldsplit Schur "upper" configuration of the reference
+ * (ptatin_driver_linear_ts.c, -pc_fieldsplit_schur_factorization_type upper).
+ * ptmf_mg_stokes_setup needs ptmf_mg_setup first. */
+int ptmf_mg_stokes_setup(ptmf_mg mg);
+int ptmf_mg_stokes_apply(ptmf_mg mg, const double *x, double *y);
+int ptmf_mg_stokes_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int restart,
+                          int *its, double *rnorm, double *history, int history_len);
 
 #ifdef __cplusplus
 }

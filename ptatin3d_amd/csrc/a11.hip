@@ -84,6 +84,59 @@ __device__ __forceinline__ void load_plane_eta(PlaneRaw<T> &r, const T *__restri
   for (int p = 0; p < 9; p++) r.eta[p] = etap[(long)p * stride];
 }
 
+// 1-D operators written with the exact structure of the Gauss tables (the
+// compiler may not drop multiplications by exact zeros or ones under IEEE
+// rules): B = [[e+o, m, e-o], [0, 1, 0], [e-o, m, e+o]] with e = 0.3,
+// o = sqrt(3/5)/2, m = 0.4, evaluated in even/odd form (6 operations instead
+// of 9).  interp3: y = B x; interp3t: y = B^T x.
+constexpr double kBe = 0.5 * (kB[0][0] + kB[0][2]);
+constexpr double kBo = 0.5 * (kB[0][0] - kB[0][2]);
+
+template <typename T>
+__device__ __forceinline__ void interp3(T x0, T x1, T x2, T &y0, T &y1, T &y2) {
+  const T sm = x0 + x2, df = x0 - x2;
+  const T t = (T)kB[0][1] * x1 + (T)kBe * sm;
+  y0 = t + (T)kBo * df;
+  y1 = x1;
+  y2 = t - (T)kBo * df;
+}
+
+template <typename T>
+__device__ __forceinline__ void interp3t(T x0, T x1, T x2, T &y0, T &y1, T &y2) {
+  const T sm = x0 + x2, df = x0 - x2;
+  const T t = (T)kBe * sm;
+  y0 = t + (T)kBo * df;
+  y1 = x1 + (T)kB[0][1] * sm;
+  y2 = t - (T)kBo * df;
+}
+
+// collocated derivative at quadrature point a of a 3-point line,
+// sum_r C[a][r] x_r, with the zero of the middle row skipped
+template <typename T>
+__device__ __forceinline__ T cdot(int a, T x0, T x1, T x2) {
+  if (a == 1) return (T)kC[1][2] * (x2 - x0);
+  return (T)kC[a][0] * x0 + (T)kC[a][1] * x1 + (T)kC[a][2] * x2;
+}
+
+// 1/x by the hardware reciprocal estimate refined with Newton-Raphson steps
+// r <- r + r (1 - x r) (two for double, one for float)
+template <typename T>
+__device__ __forceinline__ T recip(T x);
+template <>
+__device__ __forceinline__ double recip<double>(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+template <>
+__device__ __forceinline__ float recip<float>(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  const float e = fmaf(-x, r, 1.0f);
+  return fmaf(r, e, r);
+}
+
 // in-plane interpolation of a 3x3 node plane to the 3x3 quadrature grid:
 // V[d][3b+a] = sum_{i,j} B[a][i] B[b][j] g[d][3j+i]
 template <typename T>
@@ -92,16 +145,20 @@ __device__ __forceinline__ void interp_plane(const T g[3][9], T V[3][9]) {
   for (int d = 0; d < 3; d++) {
     T t[3][3];  // [j][a]
 #pragma unroll
-    for (int j = 0; j < 3; j++)
+    for (int j = 0; j < 3; j++) interp3<T>(g[d][3 * j], g[d][3 * j + 1], g[d][3 * j + 2], t[j][0], t[j][1], t[j][2]);
 #pragma unroll
-      for (int a = 0; a < 3; a++)
-        t[j][a] = (T)kB[a][0] * g[d][3 * j] + (T)kB[a][1] * g[d][3 * j + 1] + (T)kB[a][2] * g[d][3 * j + 2];
-#pragma unroll
-    for (int b = 0; b < 3; b++)
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-        V[d][3 * b + a] = (T)kB[b][0] * t[0][a] + (T)kB[b][1] * t[1][a] + (T)kB[b][2] * t[2][a];
+    for (int a = 0; a < 3; a++) interp3<T>(t[0][a], t[1][a], t[2][a], V[d][a], V[d][3 + a], V[d][6 + a]);
   }
+}
+
+// transposed in-plane interpolation: out[3j+i] = sum_{a,b} B[a][i] B[b][j] Y[3b+a]
+template <typename T>
+__device__ __forceinline__ void interp_plane_t(const T Y[9], T out[9]) {
+  T t[3][3];  // [b][i]
+#pragma unroll
+  for (int b = 0; b < 3; b++) interp3t<T>(Y[3 * b], Y[3 * b + 1], Y[3 * b + 2], t[b][0], t[b][1], t[b][2]);
+#pragma unroll
+  for (int i = 0; i < 3; i++) interp3t<T>(t[0][i], t[1][i], t[2][i], out[i], out[3 + i], out[6 + i]);
 }
 
 // Forward stage: in-plane interpolation of the lane's node plane, exchange
@@ -183,14 +240,12 @@ __device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq
       T M[3][3], Gu[3][3];
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        M[0][d] = (T)kC[a][0] * Xq[d][3 * b] + (T)kC[a][1] * Xq[d][3 * b + 1] +
-                  (T)kC[a][2] * Xq[d][3 * b + 2];
-        M[1][d] = (T)kC[b][0] * Xq[d][a] + (T)kC[b][1] * Xq[d][3 + a] + (T)kC[b][2] * Xq[d][6 + a];
+        M[0][d] = cdot<T>(a, Xq[d][3 * b], Xq[d][3 * b + 1], Xq[d][3 * b + 2]);
+        M[1][d] = cdot<T>(b, Xq[d][a], Xq[d][3 + a], Xq[d][6 + a]);
         M[2][d] = dz[0] * sh[0][9 * d + p][le] + dz[1] * sh[1][9 * d + p][le] +
                   dz[2] * sh[2][9 * d + p][le];
-        Gu[0][d] = (T)kC[a][0] * Uq[d][3 * b] + (T)kC[a][1] * Uq[d][3 * b + 1] +
-                   (T)kC[a][2] * Uq[d][3 * b + 2];
-        Gu[1][d] = (T)kC[b][0] * Uq[d][a] + (T)kC[b][1] * Uq[d][3 + a] + (T)kC[b][2] * Uq[d][6 + a];
+        Gu[0][d] = cdot<T>(a, Uq[d][3 * b], Uq[d][3 * b + 1], Uq[d][3 * b + 2]);
+        Gu[1][d] = cdot<T>(b, Uq[d][a], Uq[d][3 + a], Uq[d][6 + a]);
         Gu[2][d] = dz[0] * sh[0][27 + 9 * d + p][le] + dz[1] * sh[1][27 + 9 * d + p][le] +
                    dz[2] * sh[2][27 + 9 * d + p][le];
       }
@@ -207,7 +262,7 @@ __device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq
       A[2][2] = M[0][0] * M[1][1] - M[1][0] * M[0][1];
       const T det = M[0][0] * A[0][0] + M[0][1] * A[1][0] + M[0][2] * A[2][0];
       const T wq = (T)(kW[a] * kW[b]) * wz;
-      const T s = (T(2) * wq * etq) / det;
+      const T s = (T(2) * wq * etq) * recip<T>(det);
       // gh[k][l] = det * d u_l / d x_k
       T gh[3][3];
 #pragma unroll
@@ -235,8 +290,8 @@ __device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq
         const T dv2 = sg[0][l] * A[0][2] + sg[1][l] * A[1][2] + sg[2][l] * A[2][2];
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-          W[l][3 * b + r] += (T)kC[a][r] * dv0;
-          W[l][3 * r + a] += (T)kC[b][r] * dv1;
+          if (a != 1 || r != 1) W[l][3 * b + r] += (T)kC[a][r] * dv0;
+          if (b != 1 || r != 1) W[l][3 * r + a] += (T)kC[b][r] * dv1;
         }
         if (writer) sh[cp][9 * l + p][le] = dv2;  // zeta component, exchanged below
       }
@@ -276,20 +331,7 @@ __device__ __forceinline__ void plane_backward(int cp, int le, T (*sh)[kSlots][k
     }
   __syncthreads();
 #pragma unroll
-  for (int l = 0; l < 3; l++) {
-    T t[3][3];  // [b][i]
-#pragma unroll
-    for (int b = 0; b < 3; b++)
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-        t[b][i] = (T)kB[0][i] * Y[l][3 * b] + (T)kB[1][i] * Y[l][3 * b + 1] +
-                  (T)kB[2][i] * Y[l][3 * b + 2];
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-        out[l][3 * j + i] = (T)kB[0][j] * t[0][i] + (T)kB[1][j] * t[1][i] + (T)kB[2][j] * t[2][i];
-  }
+  for (int l = 0; l < 3; l++) interp_plane_t<T>(Y[l], out[l]);
 }
 
 // The element computation of one lane: element e (node plane origin `base`,
@@ -676,9 +718,8 @@ __device__ __forceinline__ void plane_qp_pa(int cp, int le, bool writer, const T
       T Gu[3][3];
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        Gu[0][d] = (T)kC[a][0] * Uq[d][3 * b] + (T)kC[a][1] * Uq[d][3 * b + 1] +
-                   (T)kC[a][2] * Uq[d][3 * b + 2];
-        Gu[1][d] = (T)kC[b][0] * Uq[d][a] + (T)kC[b][1] * Uq[d][3 + a] + (T)kC[b][2] * Uq[d][6 + a];
+        Gu[0][d] = cdot<T>(a, Uq[d][3 * b], Uq[d][3 * b + 1], Uq[d][3 * b + 2]);
+        Gu[1][d] = cdot<T>(b, Uq[d][a], Uq[d][3 + a], Uq[d][6 + a]);
         Gu[2][d] = dz[0] * sh[0][9 * d + p][le] + dz[1] * sh[1][9 * d + p][le] +
                    dz[2] * sh[2][9 * d + p][le];
       }
@@ -704,8 +745,8 @@ __device__ __forceinline__ void plane_qp_pa(int cp, int le, bool writer, const T
         const T dv2 = sg[0][l] * Ah[2] + sg[1][l] * Ah[5] + sg[2][l] * Ah[8];
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-          W[l][3 * b + r] += (T)kC[a][r] * dv0;
-          W[l][3 * r + a] += (T)kC[b][r] * dv1;
+          if (a != 1 || r != 1) W[l][3 * b + r] += (T)kC[a][r] * dv0;
+          if (b != 1 || r != 1) W[l][3 * r + a] += (T)kC[b][r] * dv1;
         }
         if (writer) sh[cp][9 * l + p][le] = dv2;
       }
@@ -812,20 +853,7 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
     T out[3][9];
 #pragma unroll
-    for (int l = 0; l < 3; l++) {
-      T t[3][3];
-#pragma unroll
-      for (int b = 0; b < 3; b++)
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-          t[b][i] = (T)kB[0][i] * Y[l][3 * b] + (T)kB[1][i] * Y[l][3 * b + 1] +
-                    (T)kB[2][i] * Y[l][3 * b + 2];
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-          out[l][3 * j + i] = (T)kB[0][j] * t[0][i] + (T)kB[1][j] * t[1][i] + (T)kB[2][j] * t[2][i];
-    }
+    for (int l = 0; l < 3; l++) interp_plane_t<T>(Y[l], out[l]);
     {
       T *w = win[(2 * kz + cp) % 3] + 3 * ((2 * ly) * wX + 2 * lx);
 #pragma unroll

@@ -402,6 +402,14 @@ int ptmf_a11_l1_diagonal(ptmf_op op, double *d) {
   return PTMF_OK;
 }
 
+int ptmf_schur_mass_blocks(ptmf_op op, double *Me) {
+  PTMF_TRY(check_ready(op, true, false));
+  PTMF_CHECK(Me, PTMF_ERR_ARG, "schur_mass_blocks: null");
+  PTMF_TRY(ptmf::launch_schur_blocks(op, Me, nullptr));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
 int ptmf_a12_mult(ptmf_op op, const double *p, double *yu) {
   PTMF_TRY(check_ready(op, false, true));
   PTMF_CHECK(p && yu, PTMF_ERR_ARG, "a12_mult: null");

@@ -296,6 +296,89 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// ---- Schur complement approximation: per element the 4x4 block
+// M_e[i][j] = -sum_q w detJ / eta_q NIp_i NIp_j of the scaled viscous
+// pressure mass matrix (MatAssemble_StokesPC_ScaledMassMatrix,
+// stokes_assembly.c:299-450); Minv_e its inverse (Gauss-Jordan with partial
+// pivoting).  Either output may be null.
+__global__ void __launch_bounds__(64)
+    schur_blocks_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
+                        const double *__restrict__ eta, double *__restrict__ Me,
+                        double *__restrict__ Minv) {
+  __shared__ double sx[kEPB][27][3];
+  __shared__ double sr[kEPB][32][10];
+  const int h = threadIdx.x >> 5, t = threadIdx.x & 31;
+  long e = (long)blockIdx.x * kEPB + h;
+  const bool live = e < nel;
+  if (!live) e = nel - 1;
+  if (t < 27) {
+    const long node = elem_node(mx, my, X, XY, e, t);
+    for (int d = 0; d < 3; d++) sx[h][t][d] = xc[3 * node + d];
+  }
+  __syncthreads();
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (t < 27) {
+    QpGeom g;
+    qp_geometry(sx[h], t, g);
+    double Np[4];
+    p1_basis(sx[h], g.xq, Np);
+    const double fac = -g.wdet / eta[(long)t * nel + e];
+    int k = 0;
+    for (int i = 0; i < 4; i++)
+      for (int j = i; j < 4; j++) v[k++] = fac * Np[i] * Np[j];
+  }
+  for (int k = 0; k < 10; k++) sr[h][t][k] = v[k];
+  __syncthreads();
+  if (t == 0 && live) {
+    double A[4][4];
+    int k = 0;
+    for (int i = 0; i < 4; i++)
+      for (int j = i; j < 4; j++) {
+        double s = 0.0;
+        for (int q = 0; q < 27; q++) s += sr[h][q][k];
+        A[i][j] = A[j][i] = s;
+        k++;
+      }
+    if (Me)
+      for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) Me[16 * e + 4 * i + j] = A[i][j];
+    if (Minv) {
+      double B[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+      for (int c = 0; c < 4; c++) {
+        int pr = c;
+        for (int r = c + 1; r < 4; r++)
+          if (fabs(A[r][c]) > fabs(A[pr][c])) pr = r;
+        for (int j = 0; j < 4; j++) {
+          const double ta = A[c][j], tb = B[c][j];
+          A[c][j] = A[pr][j]; B[c][j] = B[pr][j];
+          A[pr][j] = ta; B[pr][j] = tb;
+        }
+        const double ip = 1.0 / A[c][c];
+        for (int j = 0; j < 4; j++) { A[c][j] *= ip; B[c][j] *= ip; }
+        for (int r = 0; r < 4; r++) {
+          if (r == c) continue;
+          const double f = A[r][c];
+          for (int j = 0; j < 4; j++) { A[r][j] -= f * A[c][j]; B[r][j] -= f * B[c][j]; }
+        }
+      }
+      for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) Minv[16 * e + 4 * i + j] = B[i][j];
+    }
+  }
+}
+
+// yp[4e+i] = sum_j Minv_e[i][j] rp[4e+j] (+ beta yp when beta != 0)
+__global__ void schur_apply_kernel(long nel, const double *__restrict__ Minv, const double *__restrict__ rp,
+                                   double *__restrict__ yp) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < 4 * nel; t += (long)gridDim.x * blockDim.x) {
+    const long e = t >> 2;
+    const int i = (int)(t & 3);
+    const double *M = Minv + 16 * e + 4 * i;
+    const double *r = rp + 4 * e;
+    yp[t] = M[0] * r[0] + M[1] * r[1] + M[2] * r[2] + M[3] * r[3];
+  }
+}
+
 int init_constants() {
   static bool done = false;
   if (done) return PTMF_OK;
@@ -324,6 +407,25 @@ int launch_a11_l1(const ptmf_op_s *op, double *d) {
   PTMF_CHECK(b.nel < (1L << 31), PTMF_ERR_ARG, "a11_l1: mesh too large");
   a11_l1_kernel<<<dim3((unsigned)b.nel), dim3(128), 0, op->stream>>>(
       b.mx, b.my, b.nel, b.X, b.X * b.Y, op->d64.coords, op->d64.eta, op->bcnode, d);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+int launch_schur_blocks(const ptmf_op_s *op, double *Me, double *Minv) {
+  PTMF_TRY(init_constants());
+  const Box &b = op->box;
+  const long nblk = (b.nel + kEPB - 1) / kEPB;
+  schur_blocks_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
+      b.mx, b.my, b.nel, b.X, b.X * b.Y, op->d64.coords, op->d64.eta, Me, Minv);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+int launch_schur_apply(const ptmf_op_s *op, const double *Minv, const double *rp, double *yp) {
+  const long n = 4 * op->box.nel;
+  long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  schur_apply_kernel<<<dim3((unsigned)nb), dim3(256), 0, op->stream>>>(op->box.nel, Minv, rp, yp);
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }

@@ -520,6 +520,32 @@ struct MGLevel {
 
 }  // namespace ptmf
 
+namespace ptmf {
+// FGMRES basis: V (restart+1 vectors), Z (restart preconditioned vectors),
+// a work vector, and device arrays of the vector pointers for the fused
+// multi-vector kernels.
+struct KrylovSpace {
+  int restart = 0;
+  long n = 0;
+  std::vector<double *> V, Z;
+  double *w = nullptr;
+  const double **dV = nullptr, **dZ = nullptr;
+  void release() {
+    for (double *p : V) (void)hipFree(p);
+    for (double *p : Z) (void)hipFree(p);
+    (void)hipFree(w);
+    (void)hipFree((void *)dV);
+    (void)hipFree((void *)dZ);
+    V.clear();
+    Z.clear();
+    w = nullptr;
+    dV = dZ = nullptr;
+    restart = 0;
+    n = 0;
+  }
+};
+}  // namespace ptmf
+
 struct ptmf_mg_s {
   int L = 0;
   ptmf::MGLevel lev[16];
@@ -528,17 +554,18 @@ struct ptmf_mg_s {
   hipStream_t stream = nullptr;
   // reductions
   double *red_part = nullptr, *red_out = nullptr, *red_host = nullptr;
-  const double **dV = nullptr, **dZ = nullptr;  // device arrays of Krylov vector pointers
   double *coef_dev = nullptr, *coef_host = nullptr;  // Krylov coefficients (pinned upload)
   // dense coarse factor
   rocblas_handle blas = nullptr;
   double *Adense = nullptr;
   int ndense = 0;
   int *dinfo = nullptr;
-  // Krylov workspace (finest level, FP64)
-  int restart = 0;
-  std::vector<double *> V, Z;
-  double *w = nullptr, *pcg[4] = {nullptr, nullptr, nullptr, nullptr};
+  // Krylov workspaces (finest level, FP64): velocity block and saddle point
+  ptmf::KrylovSpace ku, ks;
+  double *pcg[4] = {nullptr, nullptr, nullptr, nullptr};
+  // saddle point: inverse Schur blocks (16 per element) and a velocity temporary
+  double *sMinv = nullptr, *stmp = nullptr;
+  bool stokes_ready = false;
   double *fcg[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // flexible CG workspace
   std::vector<double> history;
 };
@@ -927,6 +954,137 @@ int assemble_dense(ptmf_mg_s *mg) {
   return PTMF_OK;
 }
 
+// Right-preconditioned flexible GMRES(restart) on vectors of length n with
+// classical Gram-Schmidt applied twice (fused multi-vector dot and update
+// kernels, coefficients on the host).  apply(v, y): y = K v; prec(r, z):
+// z ~ K^-1 r (may vary between calls).  The true residual is recomputed at
+// every restart and at the end; stops when it (or the Arnoldi estimate) drops
+// below rtol * ||b||.
+template <typename Apply, typename Prec>
+int fgmres_core(ptmf_mg_s *mg, KrylovSpace &ks, long n, Apply apply, Prec prec, const double *b, double *x,
+                double rtol, int maxit, int restart, int *its_out, double *rnorm_out, double *history,
+                int history_len) {
+  hipStream_t s = mg->stream;
+  if (ks.restart != restart || ks.n != n) {
+    PTMF_HIP(hipStreamSynchronize(s));
+    ks.release();
+    ks.V.assign(restart + 1, nullptr);
+    ks.Z.assign(restart, nullptr);
+    for (auto &p : ks.V) PTMF_TRY(dalloc((void **)&p, sizeof(double) * n));
+    for (auto &p : ks.Z) PTMF_TRY(dalloc((void **)&p, sizeof(double) * n));
+    PTMF_TRY(dalloc((void **)&ks.w, sizeof(double) * n));
+    PTMF_TRY(dalloc((void **)&ks.dV, sizeof(double *) * (restart + 1)));
+    PTMF_TRY(dalloc((void **)&ks.dZ, sizeof(double *) * restart));
+    PTMF_HIP(hipMemcpy((void *)ks.dV, ks.V.data(), sizeof(double *) * (restart + 1), hipMemcpyHostToDevice));
+    PTMF_HIP(hipMemcpy((void *)ks.dZ, ks.Z.data(), sizeof(double *) * restart, hipMemcpyHostToDevice));
+    ks.restart = restart;
+    ks.n = n;
+  }
+  double *w = ks.w;
+  std::vector<double> H((restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart);
+  double bnorm2;
+  PTMF_TRY(dot(mg, n, b, b, &bnorm2));
+  const double bnorm = sqrt(bnorm2);
+  int its = 0, hl = 0;
+  double rnorm = 0.0;
+  if (bnorm == 0.0) {
+    PTMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    PTMF_HIP(hipStreamSynchronize(s));
+    if (its_out) *its_out = 0;
+    if (rnorm_out) *rnorm_out = 0.0;
+    return PTMF_OK;
+  }
+  while (true) {
+    PTMF_TRY(apply(x, w));
+    k_residual<double><<<nblocks(n), kBS, 0, s>>>(n, b, w, ks.V[0]);
+    PTMF_HIP(hipGetLastError());
+    double rr;
+    PTMF_TRY(dot(mg, n, ks.V[0], ks.V[0], &rr));
+    rnorm = sqrt(rr);
+    PTMF_CHECK(isfinite(rnorm), PTMF_ERR_DIVERGED, "fgmres: non-finite residual");
+    if (history && hl < history_len) history[hl++] = rnorm;
+    if (rnorm <= rtol * bnorm || its >= maxit) break;
+    PTMF_TRY(scale<double>(s, n, 1.0 / rnorm, ks.V[0], ks.V[0]));
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = rnorm;
+    int j = 0;
+    while (j < restart && its < maxit) {
+      PTMF_TRY(prec(ks.V[j], ks.Z[j]));
+      PTMF_TRY(apply(ks.Z[j], w));
+      double h[kMaxDots], h2[kMaxDots];
+      PTMF_TRY(mdot(mg, n, j + 1, ks.dV, w, h));
+      k_maxpy<<<nblocks(n), kBS, 0, s>>>(n, j + 1, ks.dV, mg->red_out, w);
+      PTMF_HIP(hipGetLastError());
+      PTMF_TRY(mdot(mg, n, j + 1, ks.dV, w, h2));
+      k_maxpy<<<nblocks(n), kBS, 0, s>>>(n, j + 1, ks.dV, mg->red_out, w);
+      PTMF_HIP(hipGetLastError());
+      for (int i = 0; i <= j; i++) h[i] += h2[i];
+      double ww;
+      PTMF_TRY(dot(mg, n, w, w, &ww));
+      const double hn = sqrt(ww);
+      PTMF_CHECK(isfinite(hn), PTMF_ERR_DIVERGED, "fgmres: non-finite Arnoldi vector");
+      for (int i = 0; i < j; i++) {
+        const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+        h[i] = t;
+      }
+      const double den = sqrt(h[j] * h[j] + hn * hn);
+      cs[j] = den > 0 ? h[j] / den : 1.0;
+      sn[j] = den > 0 ? hn / den : 0.0;
+      h[j] = den;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int i = 0; i <= j; i++) H[i * restart + j] = h[i];
+      its++;
+      j++;
+      if (history && hl < history_len) history[hl++] = fabs(g[j]);
+      if (fabs(g[j]) <= rtol * bnorm || hn == 0.0) break;
+      PTMF_TRY(scale<double>(s, n, 1.0 / hn, w, ks.V[j]));
+    }
+    for (int i = j - 1; i >= 0; i--) {
+      double t = g[i];
+      for (int k = i + 1; k < j; k++) t -= H[i * restart + k] * y[k];
+      y[i] = t / H[i * restart + i];
+    }
+    memcpy(mg->coef_host, y.data(), sizeof(double) * j);
+    PTMF_HIP(hipMemcpyAsync(mg->coef_dev, mg->coef_host, sizeof(double) * j, hipMemcpyHostToDevice, s));
+    k_mupdate<<<nblocks(n), kBS, 0, s>>>(n, j, ks.dZ, mg->coef_dev, x);
+    PTMF_HIP(hipGetLastError());
+  }
+  PTMF_HIP(hipStreamSynchronize(s));
+  if (its_out) *its_out = its;
+  if (rnorm_out) *rnorm_out = rnorm;
+  return PTMF_OK;
+}
+
+// Saddle point K = [A11 A12; A21 0] on [u | p] (3N + 4E values):
+// yu = A11 xu + A12 xp with Dirichlet rows y = x, yp = A21 xu
+// (MatMult of the reference's MatNest Stokes operator, stokes_operators.c).
+int stokes_apply(ptmf_mg_s *mg, const double *x, double *y) {
+  MGLevel &F = mg->lev[mg->L - 1];
+  const long nu = F.n;
+  PTMF_TRY(launch_a11_mult<double>(F.op, x, y, true));
+  PTMF_TRY(launch_a12(F.op, x + nu, y));
+  PTMF_TRY(launch_bc_copy<double>(F.op->bcidx, F.op->nbc, x, y, mg->stream));
+  PTMF_TRY(launch_a21(F.op, x, y + nu));
+  return PTMF_OK;
+}
+
+// Upper block-triangular preconditioner [A11 A12; 0 S]^-1 with S the scaled
+// viscous pressure mass (block diagonal) and A11^-1 one V-cycle:
+// zp = S^-1 rp, zu = V-cycle(ru - A12 zp)  (the fieldsplit Schur "upper"
+// factorisation of the reference's solver configuration).
+int stokes_prec(ptmf_mg_s *mg, const double *r, double *z) {
+  MGLevel &F = mg->lev[mg->L - 1];
+  const long nu = F.n;
+  PTMF_TRY(launch_schur_apply(F.op, mg->sMinv, r + nu, z + nu));
+  PTMF_TRY(launch_fill<double>(mg->stmp, 0.0, nu, mg->stream));
+  PTMF_TRY(launch_a12(F.op, z + nu, mg->stmp));
+  PTMF_TRY(launch_bc_set<double>(F.op->bcidx, F.op->nbc, 0.0, mg->stmp, mg->stream));
+  PTMF_TRY(axpby<double>(mg->stream, nu, 1.0, r, -1.0, mg->stmp));
+  return ptmf_mg_vcycle(mg, mg->stmp, z);
+}
+
 void set_op_stream(ptmf_op op, hipStream_t s) {
   if (op->stream && op->stream != s) (void)hipStreamDestroy(op->stream);
   op->stream = s;
@@ -994,8 +1152,6 @@ int ptmf_mg_create(int mx, int my, int mz, int nlevels, ptmf_mg *out) {
   }
   if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->red_part, sizeof(double) * ptmf::kRedBlocks * ptmf::kMaxDots);
   if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->red_out, sizeof(double) * ptmf::kMaxDots);
-  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->dV, sizeof(double *) * ptmf::kMaxDots);
-  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->dZ, sizeof(double *) * ptmf::kMaxDots);
   if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->coef_dev, sizeof(double) * ptmf::kMaxDots);
   if (st == PTMF_OK) {
     he = hipHostMalloc((void **)&mg->red_host, sizeof(double) * ptmf::kMaxDots, hipHostMallocDefault);
@@ -1031,15 +1187,14 @@ int ptmf_mg_destroy(ptmf_mg *pmg) {
                   L.dinv32, L.b32, L.x32, L.r32, L.p32[0], L.p32[1], L.p32[2], L.S64, L.S32};
     for (void *p : ps) (void)hipFree(p);
   }
-  for (double *p : mg->V) (void)hipFree(p);
-  for (double *p : mg->Z) (void)hipFree(p);
-  (void)hipFree(mg->w);
+  mg->ku.release();
+  mg->ks.release();
+  (void)hipFree(mg->sMinv);
+  (void)hipFree(mg->stmp);
   for (double *p : mg->pcg) (void)hipFree(p);
   for (double *p : mg->fcg) (void)hipFree(p);
   (void)hipFree(mg->red_part);
   (void)hipFree(mg->red_out);
-  (void)hipFree(mg->dV);
-  (void)hipFree(mg->dZ);
   (void)hipFree(mg->coef_dev);
   if (mg->coef_host) (void)hipHostFree(mg->coef_host);
   (void)hipFree(mg->Adense);
@@ -1279,109 +1434,16 @@ int ptmf_mg_smooth(ptmf_mg mg, int level, int precision, int its, int nonzero_gu
   return PTMF_OK;
 }
 
-// Right-preconditioned FGMRES(restart) on the finest-level A11 (FP64) with the
-// V-cycle as (variable) preconditioner.  Converged when the FGMRES residual
-// drops below rtol * ||b||; the true FP64 residual is recomputed at every
-// restart and at the end.
 int ptmf_mg_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int restart,
                    int *its_out, double *rnorm_out, double *history, int history_len) {
   PTMF_CHECK(mg && b && x && restart > 0 && restart < ptmf::kMaxDots && maxit > 0, PTMF_ERR_ARG,
              "mg_fgmres: bad arguments");
   PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_fgmres: not set up");
   MGLevel &F = mg->lev[mg->L - 1];
-  const long n = F.n;
-  hipStream_t s = mg->stream;
-  if (mg->restart != restart) {
-    PTMF_HIP(hipStreamSynchronize(s));
-    for (double *p : mg->V) (void)hipFree(p);
-    for (double *p : mg->Z) (void)hipFree(p);
-    mg->V.assign(restart + 1, nullptr);
-    mg->Z.assign(restart, nullptr);
-    for (auto &p : mg->V) PTMF_TRY(ptmf::dalloc((void **)&p, sizeof(double) * n));
-    for (auto &p : mg->Z) PTMF_TRY(ptmf::dalloc((void **)&p, sizeof(double) * n));
-    if (!mg->w) PTMF_TRY(ptmf::dalloc((void **)&mg->w, sizeof(double) * n));
-    PTMF_HIP(hipMemcpy(mg->dV, mg->V.data(), sizeof(double *) * (restart + 1), hipMemcpyHostToDevice));
-    PTMF_HIP(hipMemcpy(mg->dZ, mg->Z.data(), sizeof(double *) * restart, hipMemcpyHostToDevice));
-    mg->restart = restart;
-  }
-  double *w = mg->w;
-  std::vector<double> H((restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart);
-  double bnorm2;
-  PTMF_TRY(ptmf::dot(mg, n, b, b, &bnorm2));
-  const double bnorm = sqrt(bnorm2);
-  int its = 0, hl = 0;
-  double rnorm = 0.0;
-  if (bnorm == 0.0) {
-    PTMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
-    PTMF_HIP(hipStreamSynchronize(s));
-    if (its_out) *its_out = 0;
-    if (rnorm_out) *rnorm_out = 0.0;
-    return PTMF_OK;
-  }
-  while (true) {
-    // true residual r = b - A x into V0
-    PTMF_TRY(ptmf::matmult<double>(F, x, w));
-    ptmf::k_residual<double><<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, b, w, mg->V[0]);
-    PTMF_HIP(hipGetLastError());
-    double rr;
-    PTMF_TRY(ptmf::dot(mg, n, mg->V[0], mg->V[0], &rr));
-    rnorm = sqrt(rr);
-    PTMF_CHECK(isfinite(rnorm), PTMF_ERR_DIVERGED, "fgmres: non-finite residual");
-    if (history && hl < history_len) history[hl++] = rnorm;
-    if (rnorm <= rtol * bnorm || its >= maxit) break;
-    PTMF_TRY(ptmf::scale<double>(s, n, 1.0 / rnorm, mg->V[0], mg->V[0]));
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = rnorm;
-    int j = 0;
-    while (j < restart && its < maxit) {
-      PTMF_TRY(ptmf_mg_vcycle(mg, mg->V[j], mg->Z[j]));
-      PTMF_TRY(ptmf::matmult<double>(F, mg->Z[j], w));
-      // classical Gram-Schmidt, applied twice; the dots stay on the device
-      double h[ptmf::kMaxDots], h2[ptmf::kMaxDots];
-      PTMF_TRY(ptmf::mdot(mg, n, j + 1, mg->dV, w, h));
-      ptmf::k_maxpy<<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, j + 1, mg->dV, mg->red_out, w);
-      PTMF_HIP(hipGetLastError());
-      PTMF_TRY(ptmf::mdot(mg, n, j + 1, mg->dV, w, h2));
-      ptmf::k_maxpy<<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, j + 1, mg->dV, mg->red_out, w);
-      PTMF_HIP(hipGetLastError());
-      for (int i = 0; i <= j; i++) h[i] += h2[i];
-      double ww;
-      PTMF_TRY(ptmf::dot(mg, n, w, w, &ww));
-      const double hn = sqrt(ww);
-      PTMF_CHECK(isfinite(hn), PTMF_ERR_DIVERGED, "fgmres: non-finite Arnoldi vector");
-      for (int i = 0; i < j; i++) {
-        const double t = cs[i] * h[i] + sn[i] * h[i + 1];
-        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-        h[i] = t;
-      }
-      const double den = sqrt(h[j] * h[j] + hn * hn);
-      cs[j] = den > 0 ? h[j] / den : 1.0;
-      sn[j] = den > 0 ? hn / den : 0.0;
-      h[j] = den;
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
-      for (int i = 0; i <= j; i++) H[i * restart + j] = h[i];
-      its++;
-      j++;
-      if (history && hl < history_len) history[hl++] = fabs(g[j]);
-      if (fabs(g[j]) <= rtol * bnorm || hn == 0.0) break;
-      PTMF_TRY(ptmf::scale<double>(s, n, 1.0 / hn, w, mg->V[j]));
-    }
-    // solve the triangular system and update x += Z y
-    for (int i = j - 1; i >= 0; i--) {
-      double t = g[i];
-      for (int k = i + 1; k < j; k++) t -= H[i * restart + k] * y[k];
-      y[i] = t / H[i * restart + i];
-    }
-    memcpy(mg->coef_host, y.data(), sizeof(double) * j);
-    PTMF_HIP(hipMemcpyAsync(mg->coef_dev, mg->coef_host, sizeof(double) * j, hipMemcpyHostToDevice, s));
-    ptmf::k_mupdate<<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, j, mg->dZ, mg->coef_dev, x);
-    PTMF_HIP(hipGetLastError());
-  }
-  PTMF_HIP(hipStreamSynchronize(s));
-  if (its_out) *its_out = its;
-  if (rnorm_out) *rnorm_out = rnorm;
-  return PTMF_OK;
+  auto apply = [&](const double *v, double *y) { return ptmf::matmult<double>(F, v, y); };
+  auto prec = [&](const double *r, double *z) { return ptmf_mg_vcycle(mg, r, z); };
+  return ptmf::fgmres_core(mg, mg->ku, F.n, apply, prec, b, x, rtol, maxit, restart, its_out, rnorm_out,
+                           history, history_len);
 }
 
 
@@ -1454,6 +1516,45 @@ int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, 
   PTMF_HIP(hipStreamSynchronize(s));
   if (its_out) *its_out = its;
   if (rnorm_out) *rnorm_out = rnorm;
+  return PTMF_OK;
+}
+
+
+// Saddle-point set-up on the finest level: inverse Schur blocks of the
+// scaled viscous mass (needs the multigrid set up for the velocity block).
+int ptmf_mg_stokes_setup(ptmf_mg mg) {
+  PTMF_CHECK(mg, PTMF_ERR_ARG, "mg_stokes_setup: null");
+  PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_stokes_setup: multigrid not set up");
+  MGLevel &F = mg->lev[mg->L - 1];
+  if (!mg->sMinv) PTMF_TRY(ptmf::dalloc((void **)&mg->sMinv, sizeof(double) * 16 * F.op->box.nel));
+  if (!mg->stmp) PTMF_TRY(ptmf::dalloc((void **)&mg->stmp, sizeof(double) * F.n));
+  PTMF_TRY(ptmf::launch_schur_blocks(F.op, nullptr, mg->sMinv));
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  mg->stokes_ready = true;
+  return PTMF_OK;
+}
+
+// FGMRES on the saddle point with the block-triangular preconditioner;
+// b and x are [u (3N) | p (4E)] device vectors.
+int ptmf_mg_stokes_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int restart,
+                          int *its_out, double *rnorm_out, double *history, int history_len) {
+  PTMF_CHECK(mg && b && x && restart > 0 && restart < ptmf::kMaxDots && maxit > 0, PTMF_ERR_ARG,
+             "mg_stokes_fgmres: bad arguments");
+  PTMF_CHECK(mg->stokes_ready && mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_stokes_fgmres: not set up");
+  MGLevel &F = mg->lev[mg->L - 1];
+  const long n = F.n + 4 * F.op->box.nel;
+  auto apply = [&](const double *v, double *y) { return ptmf::stokes_apply(mg, v, y); };
+  auto prec = [&](const double *r, double *z) { return ptmf::stokes_prec(mg, r, z); };
+  return ptmf::fgmres_core(mg, mg->ks, n, apply, prec, b, x, rtol, maxit, restart, its_out, rnorm_out,
+                           history, history_len);
+}
+
+// y = K x for the saddle point ([u | p] device vectors)
+int ptmf_mg_stokes_apply(ptmf_mg mg, const double *x, double *y) {
+  PTMF_CHECK(mg && x && y, PTMF_ERR_ARG, "mg_stokes_apply: null");
+  PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_stokes_apply: not set up");
+  PTMF_TRY(ptmf::stokes_apply(mg, x, y));
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
   return PTMF_OK;
 }
 

@@ -146,6 +146,8 @@ int launch_bc_set(const long *idx, long n, T v, T *y, hipStream_t s);
 int launch_copy16(const void *x, void *y, long n16, hipStream_t s);
 int launch_a11_diag(const ptmf_op_s *op, double *d);
 int launch_a11_l1(const ptmf_op_s *op, double *d);
+int launch_schur_blocks(const ptmf_op_s *op, double *Me, double *Minv);
+int launch_schur_apply(const ptmf_op_s *op, const double *Minv, const double *rp, double *yp);
 int launch_a12(const ptmf_op_s *op, const double *p, double *yu);
 int launch_a21(const ptmf_op_s *op, const double *u, double *yp);
 }  // namespace ptmf

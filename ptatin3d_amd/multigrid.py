@@ -127,6 +127,23 @@ class GMG:
         nz = np.flatnonzero(hist)
         return its.value, rn.value, hist[: (nz[-1] + 1) if nz.size else 0]
 
+    # -- saddle point -------------------------------------------------------
+    def stokes_setup(self) -> None:
+        call("ptmf_mg_stokes_setup", self.handle)
+
+    def stokes_apply(self, x: DeviceArray, y: DeviceArray) -> None:
+        call("ptmf_mg_stokes_apply", self.handle, x.ptr, y.ptr)
+
+    def stokes_fgmres(self, b: DeviceArray, x: DeviceArray, rtol=1e-8, maxit=500, restart=50,
+                      history_len=1024):
+        its = C.c_int(0)
+        rn = C.c_double(0)
+        hist = np.zeros(history_len)
+        call("ptmf_mg_stokes_fgmres", self.handle, b.ptr, x.ptr, float(rtol), int(maxit), int(restart),
+             C.byref(its), C.byref(rn), hist.ctypes.data_as(C.POINTER(C.c_double)), int(history_len))
+        nz = np.flatnonzero(hist)
+        return its.value, rn.value, hist[: (nz[-1] + 1) if nz.size else 0]
+
     def destroy(self) -> None:
         if self.handle is not None and self.handle.value:
             call("ptmf_mg_destroy", C.byref(self.handle))
@@ -158,3 +175,21 @@ def solve_timed(mg: GMG, b: np.ndarray, rtol=1e-8, maxit=200, restart=30, krylov
     db.free()
     dx.free()
     return x, its, rn, hist, t1 - t0
+
+
+def stokes_solve_timed(mg: GMG, bu: np.ndarray, bp: np.ndarray, rtol=1e-8, maxit=500, restart=50):
+    """Saddle-point solve from x = 0; returns (u, p, its, rnorm, history, seconds)."""
+    nu = bu.size
+    b = np.concatenate([bu, bp])
+    db = DeviceArray.from_host(b)
+    dx = DeviceArray(b.size)
+    dx.zero()
+    call("ptmf_device_synchronize")
+    t0 = time.perf_counter()
+    its, rn, hist = mg.stokes_fgmres(db, dx, rtol, maxit, restart)
+    call("ptmf_device_synchronize")
+    t1 = time.perf_counter()
+    x = dx.download()
+    db.free()
+    dx.free()
+    return x[:nu], x[nu:], its, rn, hist, t1 - t0

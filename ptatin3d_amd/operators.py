@@ -187,3 +187,12 @@ class MatStokesMF(MatA11MF):
         du.free()
         dy.free()
         return out
+
+    def schur_blocks_host(self) -> np.ndarray:
+        """Element blocks of the scaled viscous pressure mass matrix, [E, 4, 4]
+        (MatAssemble_StokesPC_ScaledMassMatrix, stokes_assembly.c:299-450)."""
+        d = DeviceArray(16 * self.nel)
+        call("ptmf_schur_mass_blocks", self.handle, d.ptr)
+        out = d.download().reshape(-1, 4, 4)
+        d.free()
+        return out
