@@ -226,8 +226,11 @@ int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, 
  * V-cycle of this multigrid (precision per ptmf_mg_options) -- the
  * fieldsplit Schur "upper" configuration of the reference
  * (ptatin_driver_linear_ts.c, -pc_fieldsplit_schur_factorization_type upper).
+ * With inner_its > 1, A11^-1 is up to inner_its flexible-CG iterations
+ * (V-cycle preconditioned) to relative tolerance inner_rtol, as the
+ * reference's inner -fieldsplit_u_ksp_type fgmres -fieldsplit_u_ksp_rtol.
  * ptmf_mg_stokes_setup needs ptmf_mg_setup first. */
-int ptmf_mg_stokes_setup(ptmf_mg mg);
+int ptmf_mg_stokes_setup(ptmf_mg mg, int inner_its, double inner_rtol);
 int ptmf_mg_stokes_apply(ptmf_mg mg, const double *x, double *y);
 int ptmf_mg_stokes_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int restart,
                           int *its, double *rnorm, double *history, int history_len);

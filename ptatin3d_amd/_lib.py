@@ -82,7 +82,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_mg_smooth": (_I, [_P, _I, _I, _I, _I, _P, _P]),
     "ptmf_mg_fgmres": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _PD, _PD, _I]),
     "ptmf_mg_fcg": (_I, [_P, _P, _P, _D, _I, C.POINTER(_I), _PD, _PD, _I]),
-    "ptmf_mg_stokes_setup": (_I, [_P]),
+    "ptmf_mg_stokes_setup": (_I, [_P, _I, _D]),
     "ptmf_mg_stokes_apply": (_I, [_P, _P, _P]),
     "ptmf_mg_stokes_fgmres": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _PD, _PD, _I]),
     "ptmf_schur_mass_blocks": (_I, [_P, _P]),

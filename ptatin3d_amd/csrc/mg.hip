@@ -566,6 +566,8 @@ struct ptmf_mg_s {
   // saddle point: inverse Schur blocks (16 per element) and a velocity temporary
   double *sMinv = nullptr, *stmp = nullptr;
   bool stokes_ready = false;
+  int s_inner_its = 1;       // A11^-1 in the preconditioner: 1 = one V-cycle, k > 1 = up to k
+  double s_inner_rtol = 0.0; // flexible-CG iterations to relative tolerance s_inner_rtol
   double *fcg[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // flexible CG workspace
   std::vector<double> history;
 };
@@ -1057,6 +1059,13 @@ int fgmres_core(ptmf_mg_s *mg, KrylovSpace &ks, long n, Apply apply, Prec prec, 
   return PTMF_OK;
 }
 
+}  // namespace
+}  // namespace ptmf
+extern "C" int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int *its_out,
+                           double *rnorm_out, double *history, int history_len);
+namespace ptmf {
+namespace {
+
 // Saddle point K = [A11 A12; A21 0] on [u | p] (3N + 4E values):
 // yu = A11 xu + A12 xp with Dirichlet rows y = x, yp = A21 xu
 // (MatMult of the reference's MatNest Stokes operator, stokes_operators.c).
@@ -1082,7 +1091,11 @@ int stokes_prec(ptmf_mg_s *mg, const double *r, double *z) {
   PTMF_TRY(launch_a12(F.op, z + nu, mg->stmp));
   PTMF_TRY(launch_bc_set<double>(F.op->bcidx, F.op->nbc, 0.0, mg->stmp, mg->stream));
   PTMF_TRY(axpby<double>(mg->stream, nu, 1.0, r, -1.0, mg->stmp));
-  return ptmf_mg_vcycle(mg, mg->stmp, z);
+  if (mg->s_inner_its <= 1) return ptmf_mg_vcycle(mg, mg->stmp, z);
+  PTMF_HIP(hipMemsetAsync(z, 0, sizeof(double) * nu, mg->stream));
+  int its = 0;
+  double rn = 0.0;
+  return ptmf_mg_fcg(mg, mg->stmp, z, mg->s_inner_rtol, mg->s_inner_its, &its, &rn, nullptr, 0);
 }
 
 void set_op_stream(ptmf_op op, hipStream_t s) {
@@ -1522,8 +1535,10 @@ int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, 
 
 // Saddle-point set-up on the finest level: inverse Schur blocks of the
 // scaled viscous mass (needs the multigrid set up for the velocity block).
-int ptmf_mg_stokes_setup(ptmf_mg mg) {
-  PTMF_CHECK(mg, PTMF_ERR_ARG, "mg_stokes_setup: null");
+int ptmf_mg_stokes_setup(ptmf_mg mg, int inner_its, double inner_rtol) {
+  PTMF_CHECK(mg && inner_its >= 0 && inner_rtol >= 0.0, PTMF_ERR_ARG, "mg_stokes_setup: bad arguments");
+  mg->s_inner_its = inner_its;
+  mg->s_inner_rtol = inner_rtol;
   PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_stokes_setup: multigrid not set up");
   MGLevel &F = mg->lev[mg->L - 1];
   if (!mg->sMinv) PTMF_TRY(ptmf::dalloc((void **)&mg->sMinv, sizeof(double) * 16 * F.op->box.nel));

@@ -128,8 +128,10 @@ class GMG:
         return its.value, rn.value, hist[: (nz[-1] + 1) if nz.size else 0]
 
     # -- saddle point -------------------------------------------------------
-    def stokes_setup(self) -> None:
-        call("ptmf_mg_stokes_setup", self.handle)
+    def stokes_setup(self, inner_its: int = 1, inner_rtol: float = 0.0) -> None:
+        """Saddle-point preconditioner; A11^-1 is one V-cycle (inner_its <= 1)
+        or up to inner_its V-cycle-preconditioned flexible-CG iterations."""
+        call("ptmf_mg_stokes_setup", self.handle, int(inner_its), float(inner_rtol))
 
     def stokes_apply(self, x: DeviceArray, y: DeviceArray) -> None:
         call("ptmf_mg_stokes_apply", self.handle, x.ptr, y.ptr)
