@@ -78,6 +78,10 @@ typedef struct {
   int smoother_diag;      /* 0: point Jacobi (diag(A), -mg_levels_pc_type jacobi); 1: l1-Jacobi
                              (sum_j |a_ij|, which bounds the spectrum of D^-1 A by 1 where
                              point Jacobi lets elements cut by viscosity jumps stretch it) */
+  int use_graph;          /* 1: replay the V-cycle as a captured HIP graph from its second call
+                             (needs levels_mg == nlevels; a new ptmf_mg_setup or
+                             ptmf_mg_set_level_eigs recaptures; changing a level operator's
+                             tiling or kernel variant after set-up needs ptmf_mg_setup) */
   int transfer;           /* 0: DMDA Q1 interpolation on the Q2 node lattice (the reference's
                              DMCreateInterpolation, ptatin_driver_linear_ts.c:989); 1: Q2
                              embedding (fine nodal values of the coarse Q2 function) */

@@ -105,6 +105,7 @@ class MGOptions(C.Structure):
         ("galerkin_levels", C.c_int),
         ("bc_transfer", C.c_int),
         ("smoother_diag", C.c_int),
+        ("use_graph", C.c_int),
         ("transfer", C.c_int),
     ]
 

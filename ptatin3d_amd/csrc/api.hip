@@ -49,8 +49,10 @@ Tiling choose_tiling(int mx, int my, int mz) {
       }
     }
   const long txy = (long)((mx + bestx - 1) / bestx) * ((my + besty - 1) / besty);
-  long zc = (long)mz * txy / 6144;
-  if (zc < 2) zc = 2;
+  // about 6000 waves per launch (measured best at 64^3: z-chunks of 2); the
+  // small multigrid levels get z-chunks of 1 to keep every SIMD busy
+  long zc = (long)mz * txy / 6000;
+  if (zc < 1) zc = 1;
   if (zc > 16) zc = 16;
   if (zc > mz) zc = mz;
   return make_tiling(mx, my, mz, bestx, besty, (int)zc);

@@ -127,6 +127,36 @@ __global__ void k_mdot_partial(long n, int j0, int nk, const double *const *__re
   }
 }
 
+// partial sums of r.r, z.r and z.q: part[j * gridDim.x + block]
+__global__ void k_dot3_partial(long n, const double *__restrict__ r, const double *__restrict__ z,
+                               const double *__restrict__ q, double *__restrict__ part) {
+  __shared__ double red[3][kBS / 64];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double ri = r[i], zi = z[i];
+    a0 += ri * ri;
+    a1 += zi * ri;
+    a2 += zi * q[i];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) {
+    a0 += __shfl_down(a0, off, 64);
+    a1 += __shfl_down(a1, off, 64);
+    a2 += __shfl_down(a2, off, 64);
+  }
+  if (lane == 0) {
+    red[0][wv] = a0;
+    red[1][wv] = a1;
+    red[2][wv] = a2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double t = 0.0;
+    for (int w = 0; w < kBS / 64; w++) t += red[threadIdx.x][w];
+    part[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
 // partial sums of one dot product <a, b>
 __global__ void k_dot_partial(long n, const double *__restrict__ a, const double *__restrict__ b,
                               double *__restrict__ part) {
@@ -405,37 +435,49 @@ __global__ void k_coarsen_eta_qp(int mxc, int myc, int mzc, int mode, const doub
 // the (r, c) block entry coupling node m to node m + off(o), off(o) in
 // [-2, 2]^3, o = (oi+2) + 5 (oj+2) + 25 (ok+2).
 // ---------------------------------------------------------------------------
+// y = S x.  25 threads per node, one per (oz, oy) offset pair with the five
+// x offsets in a loop, 16 nodes per work-group: consecutive threads read
+// consecutive nodes of one offset row (coalesced), and there are enough
+// waves to cover memory latency on the small coarse levels; the 25 partial
+// sums are reduced through LDS.
+constexpr int kStNodes = 16;
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(25 * kStNodes)
     k_stencil_apply(long X, long Y, long Z, const T *__restrict__ S, const T *__restrict__ x,
                     T *__restrict__ y) {
+  __shared__ T red[25][kStNodes][3];
   const long N = X * Y * Z;
-  for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < N; m += (long)gridDim.x * blockDim.x) {
+  const int t = threadIdx.x;
+  const int ml = t % kStNodes, g = t / kStNodes;
+  const int ok = g / 5 - 2, oj = g % 5 - 2;
+  const long m = (long)blockIdx.x * kStNodes + ml;
+  T a0 = 0, a1 = 0, a2 = 0;
+  if (m < N) {
     const long i = m % X, j = (m / X) % Y, k = m / (X * Y);
-    T a0 = 0, a1 = 0, a2 = 0;
-    for (int ok = -2; ok <= 2; ok++) {
-      const long kk = k + ok;
-      if (kk < 0 || kk >= Z) continue;
-      for (int oj = -2; oj <= 2; oj++) {
-        const long jj = j + oj;
-        if (jj < 0 || jj >= Y) continue;
+    const long kk = k + ok, jj = j + oj;
+    if (kk >= 0 && kk < Z && jj >= 0 && jj < Y) {
 #pragma unroll
-        for (int oi = -2; oi <= 2; oi++) {
-          const long ii = i + oi;
-          if (ii < 0 || ii >= X) continue;
-          const int o = (oi + 2) + 5 * (oj + 2) + 25 * (ok + 2);
-          const long n = ii + X * (jj + Y * kk);
-          const T x0 = x[3 * n], x1 = x[3 * n + 1], x2 = x[3 * n + 2];
-          const T *b = S + (long)o * 9 * N + m;
-          a0 += b[0] * x0 + b[N] * x1 + b[2 * N] * x2;
-          a1 += b[3 * N] * x0 + b[4 * N] * x1 + b[5 * N] * x2;
-          a2 += b[6 * N] * x0 + b[7 * N] * x1 + b[8 * N] * x2;
-        }
+      for (int oi = -2; oi <= 2; oi++) {
+        const long ii = i + oi;
+        if (ii < 0 || ii >= X) continue;
+        const int o = (oi + 2) + 5 * (oj + 2) + 25 * (ok + 2);
+        const long n = ii + X * (jj + Y * kk);
+        const T x0 = x[3 * n], x1 = x[3 * n + 1], x2 = x[3 * n + 2];
+        const T *b = S + (long)o * 9 * N + m;
+        a0 += b[0] * x0 + b[N] * x1 + b[2 * N] * x2;
+        a1 += b[3 * N] * x0 + b[4 * N] * x1 + b[5 * N] * x2;
+        a2 += b[6 * N] * x0 + b[7 * N] * x1 + b[8 * N] * x2;
       }
     }
-    y[3 * m] = a0;
-    y[3 * m + 1] = a1;
-    y[3 * m + 2] = a2;
+  }
+  red[g][ml][0] = a0;
+  red[g][ml][1] = a1;
+  red[g][ml][2] = a2;
+  __syncthreads();
+  if (g < 3 && m < N) {
+    T acc = 0;
+    for (int q = 0; q < 25; q++) acc += red[q][ml][g];
+    y[3 * m + g] = acc;
   }
 }
 
@@ -566,6 +608,11 @@ struct ptmf_mg_s {
   // saddle point: inverse Schur blocks (16 per element) and a velocity temporary
   double *sMinv = nullptr, *stmp = nullptr;
   bool stokes_ready = false;
+  // captured V-cycle (levels_mg == L): runs between the fixed buffers gb, gx
+  hipGraph_t vgraph = nullptr;
+  hipGraphExec_t vexec = nullptr;
+  double *gb = nullptr, *gx = nullptr;
+  int vcalls = 0;
   int s_inner_its = 1;       // A11^-1 in the preconditioner: 1 = one V-cycle, k > 1 = up to k
   double s_inner_rtol = 0.0; // flexible-CG iterations to relative tolerance s_inner_rtol
   double *fcg[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // flexible CG workspace
@@ -620,6 +667,18 @@ int dot(ptmf_mg_s *mg, long n, const double *a, const double *b, double *out) {
   return PTMF_OK;
 }
 
+// (r.r, z.r, z.q) in one pass and one host synchronisation
+int dot3(ptmf_mg_s *mg, long n, const double *r, const double *z, const double *q, double *out) {
+  k_dot3_partial<<<kRedBlocks, kBS, 0, mg->stream>>>(n, r, z, q, mg->red_part);
+  PTMF_HIP(hipGetLastError());
+  k_mdot_final<<<1, kBS, 0, mg->stream>>>(3, kRedBlocks, mg->red_part, mg->red_out);
+  PTMF_HIP(hipGetLastError());
+  PTMF_HIP(hipMemcpyAsync(mg->red_host, mg->red_out, sizeof(double) * 3, hipMemcpyDeviceToHost, mg->stream));
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  memcpy(out, mg->red_host, sizeof(double) * 3);
+  return PTMF_OK;
+}
+
 template <typename T>
 int scale(hipStream_t s, long n, T a, const T *x, T *y) {
   k_scale<T><<<nblocks(n), kBS, 0, s>>>(n, a, x, y);
@@ -632,7 +691,8 @@ int matmult(const MGLevel &L, const T *x, T *y) {
   if (!L.galerkin) return launch_a11_mult<T>(L.op, x, y, true);
   const Box &b = L.op->box;
   const T *S = sizeof(T) == 8 ? (const T *)L.S64 : (const T *)L.S32;
-  k_stencil_apply<T><<<nblocks(b.nnodes), kBS, 0, L.op->stream>>>(b.X, b.Y, b.Z, S, x, y);
+  const long nb = (b.nnodes + kStNodes - 1) / kStNodes;
+  k_stencil_apply<T><<<dim3((unsigned)nb), dim3(25 * kStNodes), 0, L.op->stream>>>(b.X, b.Y, b.Z, S, x, y);
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }
@@ -685,7 +745,7 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
   if (nonzero)
     PTMF_HIP(hipMemcpyAsync(p[km1], x, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
   else
-    PTMF_HIP(hipMemsetAsync(p[km1], 0, sizeof(T) * n, s));
+    PTMF_TRY(launch_fill<T>(p[km1], T(0), n, s));  // a kernel: the V-cycle may be captured
   if (nonzero) {
     PTMF_TRY(matmult<T>(L, x, r));
     k_cheb_first<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, x, dinv, (T)scale, p[k]);
@@ -710,11 +770,14 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
 }
 
 // dense Cholesky solve on level 0: x = A0^-1 b (FP64)
+// x = A0^-1 b with the explicit inverse (lower triangle, formed once from the
+// Cholesky factor): one symmetric matrix-vector product instead of two
+// latency-bound triangular solves per V-cycle
 int dense_solve(ptmf_mg_s *mg, const double *b, double *x) {
   const int n = mg->ndense;
-  PTMF_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
-  rocblas_status st = rocsolver_dpotrs(mg->blas, rocblas_fill_lower, n, 1, mg->Adense, n, x, n);
-  PTMF_CHECK(st == rocblas_status_success, PTMF_ERR_DEVICE, "rocsolver_dpotrs: status %d", (int)st);
+  const double one = 1.0, zero = 0.0;
+  rocblas_status st = rocblas_dsymv(mg->blas, rocblas_fill_lower, n, &one, mg->Adense, n, b, 1, &zero, x, 1);
+  PTMF_CHECK(st == rocblas_status_success, PTMF_ERR_DEVICE, "rocblas_dsymv: status %d", (int)st);
   return PTMF_OK;
 }
 
@@ -953,7 +1016,20 @@ int assemble_dense(ptmf_mg_s *mg) {
   PTMF_HIP(hipMemcpyAsync(&info, mg->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
   PTMF_HIP(hipStreamSynchronize(s));
   PTMF_CHECK(info == 0, PTMF_ERR_DIVERGED, "coarse Cholesky failed (info %d): operator not SPD", info);
+  st = rocsolver_dpotri(mg->blas, rocblas_fill_lower, (int)n, mg->Adense, (int)n, mg->dinfo);
+  PTMF_CHECK(st == rocblas_status_success, PTMF_ERR_DEVICE, "rocsolver_dpotri: status %d", (int)st);
+  PTMF_HIP(hipMemcpyAsync(&info, mg->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  PTMF_HIP(hipStreamSynchronize(s));
+  PTMF_CHECK(info == 0, PTMF_ERR_DIVERGED, "coarse inverse failed (info %d)", info);
   return PTMF_OK;
+}
+
+void drop_graph(ptmf_mg_s *mg) {
+  if (mg->vexec) (void)hipGraphExecDestroy(mg->vexec);
+  if (mg->vgraph) (void)hipGraphDestroy(mg->vgraph);
+  mg->vexec = nullptr;
+  mg->vgraph = nullptr;
+  mg->vcalls = 0;
 }
 
 // Right-preconditioned flexible GMRES(restart) on vectors of length n with
@@ -1127,6 +1203,7 @@ int ptmf_mg_default_options(ptmf_mg_options *o) {
   o->bc_transfer = 1;
   o->transfer = 0;
   o->smoother_diag = 0;
+  o->use_graph = 1;
   return PTMF_OK;
 }
 
@@ -1204,6 +1281,9 @@ int ptmf_mg_destroy(ptmf_mg *pmg) {
   mg->ks.release();
   (void)hipFree(mg->sMinv);
   (void)hipFree(mg->stmp);
+  ptmf::drop_graph(mg);
+  (void)hipFree(mg->gb);
+  (void)hipFree(mg->gx);
   for (double *p : mg->pcg) (void)hipFree(p);
   for (double *p : mg->fcg) (void)hipFree(p);
   (void)hipFree(mg->red_part);
@@ -1309,6 +1389,7 @@ int ptmf_mg_set_dirichlet(ptmf_mg mg, int level, const unsigned char *mask) {
 
 int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt) {
   PTMF_CHECK(mg, PTMF_ERR_ARG, "mg_setup: null");
+  ptmf::drop_graph(mg);
   if (opt) mg->opt = *opt;
   ptmf_mg_options &o = mg->opt;
   if (o.levels_mg <= 0 || o.levels_mg > mg->L) o.levels_mg = mg->L;
@@ -1385,23 +1466,62 @@ int ptmf_mg_set_level_eigs(ptmf_mg mg, int level, double emin, double emax) {
              "mg_set_level_eigs: bad arguments");
   mg->lev[level].emin = emin;
   mg->lev[level].emax = emax;
+  ptmf::drop_graph(mg);
   if (mg->is_setup) PTMF_TRY(ptmf::place_dirichlet(mg, mg->lev[level]));
   return PTMF_OK;
 }
 
 // x = V-cycle(b) on the finest level (FP64 in and out)
+namespace ptmf {
+namespace {
+int vcycle_body(ptmf_mg_s *mg, const double *b, double *x) {
+  const int lf = mg->L - 1, lc = mg->L - mg->opt.levels_mg;
+  MGLevel &F = mg->lev[lf];
+  if (mg->opt.smoother_precision == 64 || lf == lc) return vcycle<double>(mg, lf, lc, b, x);
+  PTMF_TRY((cast<double, float>(mg->stream, F.n, b, F.b32)));
+  PTMF_TRY(vcycle<float>(mg, lf, lc, F.b32, F.x32));
+  return cast<float, double>(mg->stream, F.n, F.x32, x);
+}
+
+// Capture one V-cycle from gb into gx as a HIP graph.  Everything the cycle
+// builds lazily (tile-ordered data) exists after the first, eager call.
+int capture_vcycle(ptmf_mg_s *mg) {
+  const long n = mg->lev[mg->L - 1].n;
+  if (!mg->gb) PTMF_TRY(dalloc((void **)&mg->gb, sizeof(double) * n));
+  if (!mg->gx) PTMF_TRY(dalloc((void **)&mg->gx, sizeof(double) * n));
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  PTMF_HIP(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeThreadLocal));
+  const int st = vcycle_body(mg, mg->gb, mg->gx);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(mg->stream, &g);
+  PTMF_TRY(st);
+  PTMF_CHECK(ce == hipSuccess && g, PTMF_ERR_DEVICE, "mg_vcycle: graph capture failed: %s", hipGetErrorString(ce));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return ptmf_set_error(PTMF_ERR_DEVICE, "mg_vcycle: graph instantiation failed: %s", hipGetErrorString(ie));
+  }
+  mg->vgraph = g;
+  mg->vexec = ex;
+  return PTMF_OK;
+}
+}  // namespace
+}  // namespace ptmf
+
+// x = V-cycle(b) on the finest level (FP64 in and out).  With use_graph (and
+// the whole hierarchy in the V-cycle) the second and later calls replay a
+// captured graph.
 int ptmf_mg_vcycle(ptmf_mg mg, const double *b, double *x) {
   PTMF_CHECK(mg && b && x, PTMF_ERR_ARG, "mg_vcycle: null");
   PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_vcycle: not set up");
-  const int lf = mg->L - 1, lc = mg->L - mg->opt.levels_mg;
-  MGLevel &F = mg->lev[lf];
-  if (mg->opt.smoother_precision == 64 || lf == lc) {
-    PTMF_TRY(ptmf::vcycle<double>(mg, lf, lc, b, x));
-  } else {
-    PTMF_TRY((ptmf::cast<double, float>(mg->stream, F.n, b, F.b32)));
-    PTMF_TRY(ptmf::vcycle<float>(mg, lf, lc, F.b32, F.x32));
-    PTMF_TRY((ptmf::cast<float, double>(mg->stream, F.n, F.x32, x)));
-  }
+  const bool graph = mg->opt.use_graph && mg->opt.levels_mg == mg->L;
+  if (!graph || mg->vcalls++ == 0) return ptmf::vcycle_body(mg, b, x);
+  if (!mg->vexec) PTMF_TRY(ptmf::capture_vcycle(mg));
+  const long n = mg->lev[mg->L - 1].n;
+  PTMF_HIP(hipMemcpyAsync(mg->gb, b, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+  PTMF_HIP(hipGraphLaunch(mg->vexec, mg->stream));
+  PTMF_HIP(hipMemcpyAsync(x, mg->gx, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
   return PTMF_OK;
 }
 
@@ -1511,16 +1631,18 @@ int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, 
       PTMF_TRY(ptmf::axpby<double>(s, n, alpha, p, 1.0, x));
       PTMF_HIP(hipMemcpyAsync(rold, r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
       PTMF_TRY(ptmf::axpby<double>(s, n, -alpha, q, 1.0, r));
-      PTMF_TRY(ptmf::dot(mg, n, r, r, &rr));
+      // z = M r, then r.r, z.r and z.r_old in one fused reduction (one host
+      // synchronisation; the convergence test uses r.r)
+      PTMF_TRY(ptmf_mg_vcycle(mg, r, z));
+      double d3[3];
+      PTMF_TRY(ptmf::dot3(mg, n, r, z, rold, d3));
       its++;
+      rr = d3[0];
       rnorm = sqrt(rr);
       PTMF_CHECK(isfinite(rnorm), PTMF_ERR_DIVERGED, "fcg: non-finite residual");
       if (history && hl < history_len) history[hl++] = rnorm;
       if (rnorm <= rtol * bnorm) break;
-      PTMF_TRY(ptmf_mg_vcycle(mg, r, z));
-      double zr, zro;
-      PTMF_TRY(ptmf::dot(mg, n, z, r, &zr));
-      PTMF_TRY(ptmf::dot(mg, n, z, rold, &zro));
+      const double zr = d3[1], zro = d3[2];
       const double beta = (zr - zro) / rz;
       rz = zr;
       PTMF_TRY(ptmf::axpby<double>(s, n, 1.0, z, beta, p));

@@ -105,6 +105,22 @@ def test_fgmres_gmg_solve(gpu, prec):
     mg.destroy()
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fcg_gmg_solve(gpu, prec):
+    """Flexible CG with the V-cycle reaches the FP64 true residual 1e-8."""
+    m = (16, 16, 16)
+    opt = default_options(smoother_precision=prec, pre_its=3, post_its=3)
+    mg, coords, eta = _hierarchy(m, 4, opt=opt)
+    bc = F.dirichlet_all_faces(*m)
+    b = np.random.default_rng(12).uniform(-1, 1, 3 * F.nnodes(*m))
+    b[bc.astype(bool)] = 0.0
+    x, its, rn, hist, secs = solve_timed(mg, b, rtol=1e-8, maxit=200, krylov="fcg")
+    r = b - O.a11_matmult(*m, coords, eta, bc, x)
+    assert np.linalg.norm(r) <= 1.5e-8 * np.linalg.norm(b), (its, np.linalg.norm(r) / np.linalg.norm(b))
+    assert its < 60, its
+    mg.destroy()
+
+
 def test_dense_coarse_only(gpu):
     """A one-level hierarchy is the exact (Cholesky) solve: one iteration."""
     m = (2, 2, 2)
@@ -216,3 +232,26 @@ def test_q2_transfer_reproduces_quadratics_and_is_adjoint(gpu):
     lhs, rhs = pa.download() @ b, a @ rb.download()
     assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
     mg.destroy()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_vcycle_graph_replays_match_eager(gpu, prec):
+    """The captured V-cycle (use_graph) reproduces the eager V-cycle on every
+    replay, with host work and synchronisation between the replays."""
+    m = (8, 8, 8)
+    mg_e, coords, eta = _hierarchy(m, 3, opt=default_options(smoother_precision=prec, use_graph=0, galerkin_levels=1))
+    mg_g, _, _ = _hierarchy(m, 3, opt=default_options(smoother_precision=prec, use_graph=1, galerkin_levels=1))
+    rng = np.random.default_rng(31)
+    n = 3 * F.nnodes(*m)
+    for k in range(5):
+        b = rng.uniform(-1, 1, n)
+        db = DeviceArray.from_host(b)
+        xe, xg = DeviceArray(n), DeviceArray(n)
+        mg_e.vcycle(db, xe)
+        mg_g.vcycle(db, xg)
+        ye, yg = xe.download(), xg.download()
+        assert np.isfinite(yg).all()
+        assert rel_inf(yg, ye) <= 1e-13, k
+        _ = np.linalg.norm(ye - b)  # eager host work between replays
+    mg_e.destroy()
+    mg_g.destroy()
