@@ -167,7 +167,7 @@ def main() -> int:
             y_cpu = O.a11_matmult(m, m, m, c["coords"], c["eta"], c["bc"], c["u"])
             nrun += 1
             el = time.perf_counter() - t0
-            if el >= args.cpu_seconds or nrun >= 50:
+            if el >= args.cpu_seconds:
                 break
         cpu_s = el / nrun
         out["cpu_baseline"] = {
@@ -199,8 +199,8 @@ def main() -> int:
 def solve_leg(m: int) -> dict:
     """Layered viscosity 1e3/1e1/1e-3 with 64 weak ellipsoids (contrast 1e6),
     smooth body force, all-face Dirichlet.  Flexible CG (FP64) preconditioned
-    by a geometric V-cycle down to 4^3 (dense Cholesky there): Chebyshev(6)
-    on [0.05, 1.1] x lambda_max of D^-1 A with point Jacobi, quadrature-point
+    by a geometric V-cycle down to 4^3 (dense Cholesky there): Chebyshev(10)
+    on [0.02, 1.1] x lambda_max of D^-1 A with point Jacobi, quadrature-point
     local viscosity means on the rediscretised levels, Galerkin operators on
     the levels of at most 8^3 elements.  The V-cycle runs in FP32 or in FP64;
     both are solved to ||b - A x|| <= 1e-8 ||b|| (FP64 residual of the FP64
@@ -216,7 +216,7 @@ def solve_leg(m: int) -> dict:
     res = {"workload": f"{m}^3 Q2 elements, layered 1e3/1e1/1e-3 + 64 weak ellipsoids (contrast 1e6), "
                        "smooth body force, all-face Dirichlet",
            "velocity_dofs": 3 * F.nnodes(m, m, m), "levels": nl, "galerkin_levels": gal,
-           "krylov": "flexible CG", "smoother": "Chebyshev(6)-Jacobi on [0.05, 1.1] lambda_max"}
+           "krylov": "flexible CG", "smoother": "Chebyshev(10)-Jacobi on [0.02, 1.1] lambda_max"}
     runs = {}
     bnorm = float(np.linalg.norm(c["b"]))
     for tag, prec, rtol in (("fp64", 64, 1e-8), ("fp32", 32, 1e-8), ("tight", 64, 1e-11)):
@@ -226,8 +226,8 @@ def solve_leg(m: int) -> dict:
         mg.coarsen_viscosity("qp-arithmetic")
         mg.set_dirichlet_all_faces()
         t0 = time.perf_counter()
-        mg.setup(default_options(smoother_precision=prec, pre_its=6, post_its=6, galerkin_levels=gal,
-                                 esteig=(0.0, 0.05, 0.0, 1.1)))
+        mg.setup(default_options(smoother_precision=prec, pre_its=10, post_its=10, galerkin_levels=gal,
+                                 esteig=(0.0, 0.02, 0.0, 1.1)))
         ts = time.perf_counter() - t0
         x, its, rn, hist, secs = solve_timed(mg, c["b"], rtol=rtol, maxit=4000, krylov="fcg")
         runs[tag] = x
