@@ -32,6 +32,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOPS_PER_ELEMENT = 17253  # stokes_operators_tensor.c:281-283
+
+# HBM bytes per launch of the FP64 MatMult kernel measured with rocprofv3
+# (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from the plain copy kernel
+# of --copy-probe), keyed by (elements per direction, tiling, kernel
+# variant); see profiles/a11_tile_kernel_fp64_64cube.md.  Reported only for
+# a matching configuration.
+MEASURED_TRAFFIC = {
+    (64, (4, 5, 2), 0): 269.6e6,
+    (64, (7, 3, 2), 0): 275.0e6,
+}
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
@@ -56,6 +66,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiling")
+    ap.add_argument("--tiling", type=int, nargs=3, metavar=("TX", "TY", "ZC"),
+                    help="fixed MatMult tiling (no autotuning)")
     ap.add_argument("--no-solve", action="store_true", help="skip the mixed-precision GMG solve leg")
     ap.add_argument("--solve-m", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true")
@@ -82,7 +94,9 @@ def main() -> int:
     A = MatA11MF(m, m, m).setup(c["coords"], c["eta"], c["bc"])
     x = DeviceArray.from_host(c["u"])
     y = DeviceArray(ndof)
-    if not args.no_tune:
+    if args.tiling:
+        A.set_tiling(*args.tiling)
+    elif not args.no_tune:
         # the on-the-fly kernels only: they do the reference's per-apply work
         autotune.tune(A, precision=64, reps=10, max_shapes=4, variants=(0, 1))
     tiling = A.get_tiling()
@@ -143,7 +157,7 @@ def main() -> int:
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(tflops / PEAK_FP64_TFLOPS, 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((m, tuple(tiling), variant)),
             "kernel": ("a11_tile_pf_kernel<double, true>" if variant == 1
                        else "a11_tile_kernel<double, true>"),
             "kernel_ms": round(ms_kernel, 5),
