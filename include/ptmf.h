@@ -52,6 +52,7 @@ extern "C" {
 
 typedef struct ptmf_op_s *ptmf_op;
 typedef struct ptmf_mg_s *ptmf_mg;
+typedef struct ptmf_batch_s *ptmf_batch;
 
 /* Options of the geometric-multigrid preconditioner of the viscous block
  * (the reference configures the same solver through PETSc options:
@@ -222,6 +223,20 @@ int ptmf_mg_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxi
  * Same stopping rule and outputs as ptmf_mg_fgmres. */
 int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int *its,
                 double *rnorm, double *history, int history_len);
+
+/* ---- batched MatMult: many independent boxes, one launch per stage ----- *
+ * Batches set-up operators of any sizes (each keeps its own coefficients,
+ * tiling and Dirichlet data); ptmf_batch_mult applies MatMult_MFStokes_A11 to
+ * every box, x and y being host arrays of nops device pointers (double** for
+ * precision 64, float** for 32).  The work of all boxes is one list of
+ * (box, tile) items, so small boxes no longer cost a launch each.  No
+ * counterpart in the reference (one PETSc Mat per problem). */
+int ptmf_batch_create(const ptmf_op *ops, int nops, ptmf_batch *batch);
+int ptmf_batch_destroy(ptmf_batch *batch);
+int ptmf_batch_mult(ptmf_batch batch, int precision, const void *const *x, void *const *y);
+/* ms per batched MatMult over reps back-to-back launches (HIP events) */
+int ptmf_batch_time(ptmf_batch batch, int precision, const void *const *x, void *const *y, int reps,
+                    double *ms_per_mult);
 
 /* ---- saddle point [A11 A12; A21 0] on the finest level ------------------ *
  * Vectors are [u (3N) | p (4E)] device arrays.  The solver is FGMRES with the

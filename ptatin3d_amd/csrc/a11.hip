@@ -34,6 +34,10 @@
 //     Dirichlet dofs.
 // The spmv path (y += A u, no boundary handling) keeps a one-pass kernel with
 // global atomic adds; it serves the host-array PETSc adapter contract.
+#include <string.h>
+
+#include <vector>
+
 #include "ptmf_internal.h"
 
 namespace ptmf {
@@ -490,14 +494,14 @@ __device__ __forceinline__ void flush_plane(const TileArgs &a, long tile, const 
   }
 }
 
+// One tile's z-march (the work of one wave); shared by the single-box and
+// the batched MatMult kernels.
 template <typename T, bool kMask>
-__global__ void __launch_bounds__(64)
-    a11_tile_kernel(TileArgs a, const T *__restrict__ xc, const T *__restrict__ eta,
-                    const uint8_t *__restrict__ bcn, const T *__restrict__ u, T *__restrict__ y,
-                    T *__restrict__ fbx, T *__restrict__ fby, T *__restrict__ fbz) {
-  __shared__ T sh[3][kSlots][kEPW];
-  __shared__ T win[3][3 * kWinNodes];
-  const long tile = xcd_tile(blockIdx.x, a.ntiles);
+__device__ __forceinline__ void tile_march(const TileArgs &a, long tile, const T *__restrict__ xc,
+                                           const T *__restrict__ eta, const uint8_t *__restrict__ bcn,
+                                           const T *__restrict__ u, T *__restrict__ y, T *__restrict__ fbx,
+                                           T *__restrict__ fby, T *__restrict__ fbz,
+                                           T (*sh)[kSlots][kEPW], T (*win)[3 * kWinNodes]) {
   const TileBox tb = tile_box(a, tile);
   const int lane = threadIdx.x;
   const int c = lane / kEPW;
@@ -544,6 +548,16 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
   }
   flush_plane<T>(a, tile, tb, 2 * (tb.z0 + tb.nzt), win[(2 * tb.nzt) % 3], y, fbx, fby, fbz);
+}
+
+template <typename T, bool kMask>
+__global__ void __launch_bounds__(64)
+    a11_tile_kernel(TileArgs a, const T *__restrict__ xc, const T *__restrict__ eta,
+                    const uint8_t *__restrict__ bcn, const T *__restrict__ u, T *__restrict__ y,
+                    T *__restrict__ fbx, T *__restrict__ fby, T *__restrict__ fbz) {
+  __shared__ T sh[3][kSlots][kEPW];
+  __shared__ T win[3][3 * kWinNodes];
+  tile_march<T, kMask>(a, xcd_tile(blockIdx.x, a.ntiles), xc, eta, bcn, u, y, fbx, fby, fbz, sh, win);
 }
 
 // The same z-march, software-pipelined across layers: the raw node plane of
@@ -882,10 +896,9 @@ __global__ void __launch_bounds__(64)
 // Adds the neighbours' face-buffer contributions into the nodes on the lower
 // faces of each tile.
 template <typename T>
-__global__ void __launch_bounds__(64)
-    a11_face_kernel(TileArgs a, const T *__restrict__ fbx,
-                    const T *__restrict__ fby, const T *__restrict__ fbz, T *__restrict__ y) {
-  const long tile = xcd_tile(blockIdx.x, a.ntiles);
+__device__ __forceinline__ void face_sum(const TileArgs &a, long tile, const T *__restrict__ fbx,
+                                         const T *__restrict__ fby, const T *__restrict__ fbz,
+                                         T *__restrict__ y) {
   const TileBox tb = tile_box(a, tile);
   const int oX = 2 * tb.nxt + (tb.xend ? 1 : 0);
   const int oY = 2 * tb.nyt + (tb.yend ? 1 : 0);
@@ -929,6 +942,61 @@ __global__ void __launch_bounds__(64)
     const long n = (long)I + a.X * (long)J + a.XY * (long)K;
 #pragma unroll
     for (int l = 0; l < 3; l++) y[3 * n + l] += acc[l];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+    a11_face_kernel(TileArgs a, const T *__restrict__ fbx, const T *__restrict__ fby,
+                    const T *__restrict__ fbz, T *__restrict__ y) {
+  face_sum<T>(a, xcd_tile(blockIdx.x, a.ntiles), fbx, fby, fbz, y);
+}
+
+// ---------------------------------------------------------------------------
+// Batched MatMult: many independent boxes in one launch per stage.  A work
+// item is (box, tile); every box has its own tiling, coefficients, Dirichlet
+// data and face buffers (BoxDesc, in device memory).
+// ---------------------------------------------------------------------------
+struct BoxDesc {
+  TileArgs a;
+  const void *xc, *eta;  // coordinates and tile-ordered viscosity of the precision
+  const uint8_t *bcn;
+  const long *bcidx;
+  long nbc;
+  void *fbx, *fby, *fbz;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+    a11_tile_batch_kernel(const BoxDesc *__restrict__ boxes, const int2 *__restrict__ work, long nwork,
+                          const T *const *__restrict__ xs, T *const *__restrict__ ys) {
+  __shared__ T sh[3][kSlots][kEPW];
+  __shared__ T win[3][3 * kWinNodes];
+  const int2 w = work[xcd_tile(blockIdx.x, nwork)];
+  const BoxDesc &d = boxes[w.x];
+  tile_march<T, true>(d.a, w.y, (const T *)d.xc, (const T *)d.eta, d.bcn, xs[w.x], ys[w.x], (T *)d.fbx,
+                      (T *)d.fby, (T *)d.fbz, sh, win);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+    a11_face_batch_kernel(const BoxDesc *__restrict__ boxes, const int2 *__restrict__ work, long nwork,
+                          T *const *__restrict__ ys) {
+  const int2 w = work[xcd_tile(blockIdx.x, nwork)];
+  const BoxDesc &d = boxes[w.x];
+  face_sum<T>(d.a, w.y, (const T *)d.fbx, (const T *)d.fby, (const T *)d.fbz, ys[w.x]);
+}
+
+// Dirichlet rows y = x of every box (blockIdx.y = box)
+template <typename T>
+__global__ void bc_copy_batch_kernel(const BoxDesc *__restrict__ boxes, const T *const *__restrict__ xs,
+                                     T *const *__restrict__ ys) {
+  const BoxDesc &d = boxes[blockIdx.y];
+  const T *x = xs[blockIdx.y];
+  T *y = ys[blockIdx.y];
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < d.nbc; k += (long)gridDim.x * blockDim.x) {
+    const long g = d.bcidx[k];
+    y[g] = x[g];
   }
 }
 
@@ -1089,3 +1157,183 @@ template int launch_a11_mult<double>(ptmf_op_s *, const double *, double *, bool
 template int launch_a11_mult<float>(ptmf_op_s *, const float *, float *, bool);
 
 }  // namespace ptmf
+
+// ---------------------------------------------------------------------------
+// Batched MatMult object (C-ABI)
+// ---------------------------------------------------------------------------
+struct ptmf_batch_s {
+  std::vector<ptmf_op> ops;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // device data, rebuilt when the operators' tilings or buffers change
+  ptmf::BoxDesc *dbox[2] = {nullptr, nullptr};
+  std::vector<unsigned char> hbox[2];
+  int2 *dwork = nullptr;
+  long nwork = 0;
+  std::vector<int> key;  // TX, TY, ZC, ntiles per operator of the current work list
+  void **dxy[2] = {nullptr, nullptr};  // device arrays of x and y pointers, per precision
+  std::vector<const void *> hxy[2];
+  long maxnbc = 0;
+};
+
+namespace {
+
+template <typename T>
+int batch_prepare(ptmf_batch_s *b, const void *const *x, void *const *y) {
+  const int ip = sizeof(T) == 8 ? 0 : 1;
+  const int n = (int)b->ops.size();
+  // work list of (box, tile)
+  std::vector<int> key;
+  for (ptmf_op op : b->ops) {
+    const ptmf::Tiling &t = op->tiling;
+    key.insert(key.end(), {t.TX, t.TY, t.ZC, (int)t.ntiles});
+  }
+  if (key != b->key) {
+    std::vector<int2> work;
+    for (int i = 0; i < n; i++)
+      for (long t = 0; t < b->ops[i]->tiling.ntiles; t++) work.push_back(make_int2(i, (int)t));
+    PTMF_CHECK(!work.empty() && work.size() < (1UL << 31), PTMF_ERR_ARG, "batch: %zu tiles", work.size());
+    (void)hipFree(b->dwork);
+    b->dwork = nullptr;
+    PTMF_HIP(hipMalloc((void **)&b->dwork, sizeof(int2) * work.size()));
+    PTMF_HIP(hipMemcpy(b->dwork, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice));
+    b->nwork = (long)work.size();
+    b->key = key;
+  }
+  // per-box descriptors of this precision
+  std::vector<ptmf::BoxDesc> box(n);
+  b->maxnbc = 0;
+  for (int i = 0; i < n; i++) {
+    ptmf_op op = b->ops[i];
+    PTMF_CHECK(op->has_coords && op->has_eta && op->has_bc, PTMF_ERR_NOT_SETUP,
+               "batch: operator %d: MF operator not setup", i);
+    PTMF_TRY(ptmf::ensure_eta_tiled<T>(op));
+    ptmf::BoxDesc &d = box[i];
+    d.a = ptmf::tile_args(op);
+    const ptmf::OpData<T> &od = *(const ptmf::OpData<T> *)(ip == 0 ? (const void *)&op->d64 : (const void *)&op->d32);
+    d.xc = od.coords;
+    d.eta = ip == 0 ? (const void *)op->et64 : (const void *)op->et32;
+    d.bcn = op->bcnode;
+    d.bcidx = op->bcidx;
+    d.nbc = op->nbc;
+    T *fb = (T *)op->faces;
+    d.fbx = fb;
+    d.fby = fb + 3 * d.a.fbx_n * d.a.ntiles;
+    d.fbz = (T *)d.fby + 3 * d.a.fby_n * d.a.ntiles;
+    if (op->nbc > b->maxnbc) b->maxnbc = op->nbc;
+  }
+  const size_t bytes = sizeof(ptmf::BoxDesc) * n;
+  if (!b->dbox[ip]) PTMF_HIP(hipMalloc((void **)&b->dbox[ip], bytes));
+  const unsigned char *raw = (const unsigned char *)box.data();
+  if (b->hbox[ip].size() != bytes || memcmp(b->hbox[ip].data(), raw, bytes) != 0) {
+    b->hbox[ip].assign(raw, raw + bytes);
+    PTMF_HIP(hipMemcpyAsync(b->dbox[ip], b->hbox[ip].data(), bytes, hipMemcpyHostToDevice, b->stream));
+  }
+  // x and y pointer arrays
+  std::vector<const void *> xy(2 * n);
+  for (int i = 0; i < n; i++) {
+    PTMF_CHECK(x[i] && y[i] && x[i] != y[i], PTMF_ERR_ARG, "batch: vectors of box %d", i);
+    xy[i] = x[i];
+    xy[n + i] = y[i];
+  }
+  if (!b->dxy[ip]) PTMF_HIP(hipMalloc((void **)&b->dxy[ip], sizeof(void *) * 2 * n));
+  if (b->hxy[ip] != xy) {
+    b->hxy[ip] = xy;
+    PTMF_HIP(hipMemcpyAsync(b->dxy[ip], b->hxy[ip].data(), sizeof(void *) * 2 * n, hipMemcpyHostToDevice,
+                            b->stream));
+  }
+  return PTMF_OK;
+}
+
+template <typename T>
+int batch_launch(ptmf_batch_s *b) {
+  const int ip = sizeof(T) == 8 ? 0 : 1;
+  const int n = (int)b->ops.size();
+  const T *const *xs = (const T *const *)b->dxy[ip];
+  T *const *ys = (T *const *)(b->dxy[ip] + n);
+  ptmf::a11_tile_batch_kernel<T><<<dim3((unsigned)b->nwork), dim3(64), 0, b->stream>>>(b->dbox[ip], b->dwork,
+                                                                                      b->nwork, xs, (T *const *)ys);
+  PTMF_HIP(hipGetLastError());
+  ptmf::a11_face_batch_kernel<T><<<dim3((unsigned)b->nwork), dim3(64), 0, b->stream>>>(b->dbox[ip], b->dwork,
+                                                                                       b->nwork, ys);
+  PTMF_HIP(hipGetLastError());
+  if (b->maxnbc > 0) {
+    long gx = (b->maxnbc + 255) / 256;
+    if (gx > 1024) gx = 1024;
+    ptmf::bc_copy_batch_kernel<T><<<dim3((unsigned)gx, (unsigned)n), dim3(256), 0, b->stream>>>(b->dbox[ip], xs, ys);
+    PTMF_HIP(hipGetLastError());
+  }
+  return PTMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ptmf_batch_create(const ptmf_op *ops, int nops, ptmf_batch *out) {
+  PTMF_CHECK(ops && out && nops > 0 && nops <= 65535, PTMF_ERR_ARG, "batch_create: bad arguments");
+  for (int i = 0; i < nops; i++) PTMF_CHECK(ops[i], PTMF_ERR_ARG, "batch_create: null operator %d", i);
+  ptmf_batch_s *b = new ptmf_batch_s;
+  b->ops.assign(ops, ops + nops);
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e != hipSuccess) {
+    ptmf_batch tmp = b;
+    ptmf_batch_destroy(&tmp);
+    return ptmf_set_error(PTMF_ERR_DEVICE, "batch_create: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return PTMF_OK;
+}
+
+int ptmf_batch_destroy(ptmf_batch *pb) {
+  if (!pb || !*pb) return PTMF_OK;
+  ptmf_batch_s *b = *pb;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (int i = 0; i < 2; i++) {
+    (void)hipFree(b->dbox[i]);
+    (void)hipFree(b->dxy[i]);
+  }
+  (void)hipFree(b->dwork);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  *pb = nullptr;
+  return PTMF_OK;
+}
+
+int ptmf_batch_mult(ptmf_batch b, int precision, const void *const *x, void *const *y) {
+  PTMF_CHECK(b && x && y, PTMF_ERR_ARG, "batch_mult: null argument");
+  PTMF_CHECK(precision == 64 || precision == 32, PTMF_ERR_UNSUPPORTED, "batch_mult: precision %d", precision);
+  for (ptmf_op op : b->ops) PTMF_HIP(hipStreamSynchronize(op->stream));
+  if (precision == 64) {
+    PTMF_TRY(batch_prepare<double>(b, x, y));
+    PTMF_TRY(batch_launch<double>(b));
+  } else {
+    PTMF_TRY(batch_prepare<float>(b, x, y));
+    PTMF_TRY(batch_launch<float>(b));
+  }
+  PTMF_HIP(hipStreamSynchronize(b->stream));
+  return PTMF_OK;
+}
+
+int ptmf_batch_time(ptmf_batch b, int precision, const void *const *x, void *const *y, int reps, double *ms) {
+  PTMF_CHECK(b && x && y && reps > 0 && ms, PTMF_ERR_ARG, "batch_time: bad arguments");
+  PTMF_TRY(ptmf_batch_mult(b, precision, x, y));  // prepares and warms up
+  PTMF_HIP(hipEventRecord(b->ev0, b->stream));
+  for (int r = 0; r < reps; r++) {
+    if (precision == 64) PTMF_TRY(batch_launch<double>(b));
+    else PTMF_TRY(batch_launch<float>(b));
+  }
+  PTMF_HIP(hipEventRecord(b->ev1, b->stream));
+  PTMF_HIP(hipEventSynchronize(b->ev1));
+  float t = 0.0f;
+  PTMF_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
+  *ms = (double)t / reps;
+  return PTMF_OK;
+}
+
+}  // extern "C"
+

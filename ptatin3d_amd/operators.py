@@ -196,3 +196,32 @@ class MatStokesMF(MatA11MF):
         out = d.download().reshape(-1, 4, 4)
         d.free()
         return out
+
+
+class BatchA11:
+    """MatMult_MFStokes_A11 of many independent boxes in one launch per stage
+    (set-up MatA11MF operators of any sizes)."""
+
+    def __init__(self, ops):
+        self.ops = list(ops)
+        arr = (C.c_void_p * len(self.ops))(*[op.handle.value for op in self.ops])
+        h = C.c_void_p()
+        call("ptmf_batch_create", arr, len(self.ops), C.byref(h))
+        self.handle = h
+
+    @staticmethod
+    def _ptrs(arrs):
+        return (C.c_void_p * len(arrs))(*[a.ptr for a in arrs])
+
+    def mult(self, xs, ys, precision: int = 64) -> None:
+        call("ptmf_batch_mult", self.handle, int(precision), self._ptrs(xs), self._ptrs(ys))
+
+    def time_mult(self, xs, ys, reps: int, precision: int = 64) -> float:
+        ms = C.c_double()
+        call("ptmf_batch_time", self.handle, int(precision), self._ptrs(xs), self._ptrs(ys), int(reps), C.byref(ms))
+        return ms.value
+
+    def destroy(self) -> None:
+        if self.handle is not None and self.handle.value:
+            call("ptmf_batch_destroy", C.byref(self.handle))
+        self.handle = None
