@@ -7,7 +7,8 @@ Dirichlet rows on all faces.  One step = one MatMult with vectors resident in
 HBM (column masking, element apply, Dirichlet rows).  Prints one JSON line.
 
 The MatMult tiling and kernel variant are picked by the autotuner before the
-timed region.  The "solve" object is the mixed-precision multigrid solve of
+timed region.  The "batch" object times the batched MatMult of --batch
+independent boxes (default 256, 0 skips it) against one MatMult per box.  The "solve" object is the mixed-precision multigrid solve of
 the layered 1e6-contrast workload (FP32 V-cycle vs FP64 V-cycle, flexible CG
 in FP64, both to a 1e-8 relative FP64 residual), at --solve-m elements per
 direction (default 64; --no-solve skips it).  --copy-probe runs one plain
@@ -71,6 +72,7 @@ def main() -> int:
     ap.add_argument("--no-solve", action="store_true", help="skip the mixed-precision GMG solve leg")
     ap.add_argument("--solve-m", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--batch", type=int, default=256, help="boxes in the batched-MatMult leg (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -204,10 +206,57 @@ def main() -> int:
         dst.free()
 
     A.destroy()
+    if args.batch > 0:
+        out["batch"] = batch_leg(args.batch)
     if not args.no_solve:
         out["solve"] = solve_leg(args.solve_m)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def batch_leg(nbox: int) -> dict:
+    """MatMult of nbox independent boxes (each side drawn from {4, 8, ..., 24}
+    elements, perturbed coordinates, log-uniform viscosity in [1e-3, 1e3],
+    seeded), one batched launch per stage vs one MatMult per box."""
+    from ptatin3d_amd import fields as F
+    from ptatin3d_amd._lib import DeviceArray, call
+    from ptatin3d_amd.operators import BatchA11, MatA11MF
+
+    rng = np.random.default_rng(20261202)
+    ops, vec = [], {64: ([], []), 32: ([], [])}
+    E = ndof = 0
+    for _ in range(nbox):
+        m = tuple(int(v) for v in rng.choice([4, 8, 12, 16, 20, 24], size=3))
+        coords = F.perturb_interior(F.box_coords(*m), *m, 0.1, rng)
+        eta = F.eta_loguniform(m[0] * m[1] * m[2], 1e-3, 1e3, rng)
+        ops.append(MatA11MF(*m).setup(coords, eta, F.dirichlet_all_faces(*m)))
+        x = rng.uniform(-1, 1, 3 * F.nnodes(*m))
+        for prec, dt in ((64, np.float64), (32, np.float32)):
+            vec[prec][0].append(DeviceArray.from_host(x.astype(dt)))
+            vec[prec][1].append(DeviceArray(x.size, dt))
+        E += m[0] * m[1] * m[2]
+        ndof += x.size
+    B = BatchA11(ops)
+    res = {"boxes": nbox, "elements": E, "velocity_dofs": ndof}
+    for prec in (64, 32):
+        xs, ys = vec[prec]
+        B.time_mult(xs, ys, 3, prec)
+        ms_b = B.time_mult(xs, ys, 20, prec)
+        for A, x, y in zip(ops, xs, ys):
+            A.mult(x, y)
+        call("ptmf_device_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(5):
+            for A, x, y in zip(ops, xs, ys):
+                A.mult(x, y)
+        call("ptmf_device_synchronize")
+        ms_s = (time.perf_counter() - t0) / 5 * 1e3
+        res[f"fp{prec}"] = {"batched_ms": round(ms_b, 4), "batched_GDoFs": round(ndof / ms_b / 1e6, 2),
+                            "per_box_ms": round(ms_s, 4), "per_box_GDoFs": round(ndof / ms_s / 1e6, 2)}
+    B.destroy()
+    for A in ops:
+        A.destroy()
+    return res
 
 
 def solve_leg(m: int) -> dict:
