@@ -116,6 +116,16 @@ int ptmf_op_set_dirichlet(ptmf_op op, const unsigned char *mask);
 /* device-array variants (no host round trip) */
 int ptmf_op_set_coordinates_device(ptmf_op op, const double *coords);
 int ptmf_op_set_viscosity_device(ptmf_op op, const double *eta);
+/* Strain-rate-dependent viscosity at the quadrature points (device arrays,
+ * element-major 27E like ptmf_op_set_viscosity):
+ *   eta = clip(B * max(eII, 1e-17)^(1/n - 1), eta_min, eta_max),
+ *   eII = sqrt(0.5 D:D), D = sym(grad u)
+ * (ComputeStrainRate3d / ComputeSecondInvariant3d and the VISCOUS_ARRHENIUS_2
+ * law, stokes_rheology_vp_std.c:85-148,437-461, evaluated at quadrature points
+ * with B = Ascale preexpA^(-1/n) exp(E/(n R T)) given per point).  Feed the
+ * result to ptmf_op_set_viscosity_device for a Picard update. */
+int ptmf_op_powerlaw_viscosity(ptmf_op op, const double *u, const double *B, double nexp, double eta_min,
+                               double eta_max, double *eta);
 /* Work decomposition of the MatMult kernel: TX x TY element tiles (TX*TY <= 21)
  * marching through ZC element layers.  A default is chosen at creation; the
  * autotuner overrides it.  No reference counterpart (the CUDA backend's

@@ -133,3 +133,11 @@ def cheb_jacobi(mx, my, mz, coords, eta, bc, diag, emin, emax, its, b, x0=None):
                            C.c_double(emin), C.c_double(emax), C.c_int(its),
                            C.c_int(0 if x0 is None else 1), _p(_f64(b)), _p(x))
     return x
+
+
+def powerlaw_eta(mx, my, mz, coords, u, B, nexp, eta_min, eta_max):
+    """Strain-rate-dependent quadrature-point viscosity, element-major [27E]."""
+    eta = np.zeros(27 * mx * my * mz)
+    load().orc_powerlaw_eta(mx, my, mz, _p(_f64(coords)), _p(_f64(u)), _p(_f64(B)), C.c_double(nexp),
+                            C.c_double(eta_min), C.c_double(eta_max), _p(eta))
+    return eta

@@ -711,3 +711,49 @@ void orc_cheb_jacobi(int mx, int my, int mz, const double *coords, const double 
   free(p[1]);
   free(p[2]);
 }
+
+/* ------------------------------------------------------------------------ */
+/* strain-rate-dependent viscosity at the quadrature points                  */
+/* ------------------------------------------------------------------------ */
+
+/* eta[27e+q] = clip(B[27e+q] * max(eII, 1e-17)^(1/n - 1), eta_min, eta_max),
+ * eII = sqrt(0.5 D:D), D = sym(grad u) at the quadrature point
+ * (ComputeStrainRate3d / ComputeSecondInvariant3d, stokes_rheology_vp_std.c:85-148;
+ * the power law of VISCOUS_ARRHENIUS_2, stokes_rheology_vp_std.c:437-461, with
+ * the prefactor B = Ascale preexpA^(-1/n) exp(E / (n R T)) given per point). */
+void orc_powerlaw_eta(int mx, int my, int mz, const double *coords, const double *u,
+                      const double *B, double nexp, double eta_min, double eta_max, double *eta)
+{
+  box_t b = mkbox(mx, my, mz);
+  static double W[27], XI[27][3], NI[27][27], GNI[27][3][27];
+  prep_q2_3x3(W, XI, NI, GNI);
+  for (long e = 0; e < b.nel; e++) {
+    double xe[81], ue[81], detJ[27], dNx[27][27], dNy[27][27], dNz[27][27];
+    gather_vec(&b, e, coords, xe);
+    gather_vec(&b, e, u, ue);
+    geom_q2(xe, GNI, detJ, dNx, dNy, dNz);
+    for (int q = 0; q < 27; q++) {
+      double exx = 0, eyy = 0, ezz = 0, exy = 0, exz = 0, eyz = 0;
+      for (int k = 0; k < 27; k++) {
+        const double ux = ue[3 * k], uy = ue[3 * k + 1], uz = ue[3 * k + 2];
+        exx += dNx[q][k] * ux;
+        eyy += dNy[q][k] * uy;
+        ezz += dNz[q][k] * uz;
+        exy += dNy[q][k] * ux + dNx[q][k] * uy;
+        exz += dNz[q][k] * ux + dNx[q][k] * uz;
+        eyz += dNz[q][k] * uy + dNy[q][k] * uz;
+      }
+      exy *= 0.5;
+      exz *= 0.5;
+      eyz *= 0.5;
+      const double sum = exx * exx + eyy * eyy + ezz * ezz + 2.0 * (exy * exy + exz * exz + eyz * eyz);
+      double sr = sqrt(0.5 * sum);
+      if (sr < 1.0e-17) sr = 1.0e-17;
+      double v = B[27 * e + q] * pow(sr, 1.0 / nexp - 1.0);
+      if (v < eta_min) v = eta_min;
+      if (v > eta_max) v = eta_max;
+      eta[27 * e + q] = v;
+    }
+  }
+}
+

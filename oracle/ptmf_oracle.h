@@ -94,6 +94,11 @@ void orc_cheb_jacobi(int mx, int my, int mz, const double *coords,
                      const double *diag, double emin, double emax, int its,
                      int nonzero_guess, const double *b, double *x);
 
+/* eta[27e+q] = clip(B[27e+q] * max(eII, 1e-17)^(1/n-1), eta_min, eta_max),
+ * eII = sqrt(0.5 D:D) of u at the quadrature points (stokes_rheology_vp_std.c:85-148,437-461) */
+void orc_powerlaw_eta(int mx, int my, int mz, const double *coords, const double *u,
+                      const double *B, double nexp, double eta_min, double eta_max, double *eta);
+
 #ifdef __cplusplus
 }
 #endif

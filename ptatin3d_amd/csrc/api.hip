@@ -292,10 +292,22 @@ int ptmf_op_set_viscosity(ptmf_op op, const double *eta) {
 
 int ptmf_op_set_viscosity_device(ptmf_op op, const double *eta) {
   PTMF_CHECK(op && eta, PTMF_ERR_ARG, "set_viscosity_device: null");
-  const long n = 27 * op->box.nel;
-  std::vector<double> h(n);
-  PTMF_HIP(hipMemcpy(h.data(), eta, sizeof(double) * n, hipMemcpyDeviceToHost));
-  return ptmf_op_set_viscosity(op, h.data());
+  PTMF_TRY(ptmf::launch_eta_qpmajor(op->box.nel, eta, op->d64.eta, op->d32.eta, op->stream));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  op->has_eta = 1;
+  op->state++;
+  return PTMF_OK;
+}
+
+int ptmf_op_powerlaw_viscosity(ptmf_op op, const double *u, const double *B, double nexp, double eta_min,
+                               double eta_max, double *eta) {
+  PTMF_CHECK(op && u && B && eta, PTMF_ERR_ARG, "powerlaw_viscosity: null");
+  PTMF_CHECK(op->has_coords, PTMF_ERR_NOT_SETUP, "powerlaw_viscosity: coordinates not set");
+  PTMF_CHECK(nexp > 0.0 && eta_min > 0.0 && eta_max >= eta_min, PTMF_ERR_ARG,
+             "powerlaw_viscosity: n %g, bounds [%g, %g]", nexp, eta_min, eta_max);
+  PTMF_TRY(ptmf::launch_powerlaw(op, u, B, nexp, eta_min, eta_max, eta));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
 }
 
 int ptmf_op_set_dirichlet(ptmf_op op, const unsigned char *mask) {

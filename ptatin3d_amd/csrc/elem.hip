@@ -379,6 +379,54 @@ __global__ void schur_apply_kernel(long nel, const double *__restrict__ Minv, co
   }
 }
 
+// ---- strain-rate-dependent viscosity at the quadrature points:
+// eta = clip(B * max(eII, 1e-17)^(1/n - 1)), eII = sqrt(0.5 D:D), D = sym(grad u)
+// (ComputeStrainRate3d / ComputeSecondInvariant3d, stokes_rheology_vp_std.c:85-148;
+// the VISCOUS_ARRHENIUS_2 power law, stokes_rheology_vp_std.c:437-461, with the
+// prefactor B per quadrature point).  Element-major input B and output eta.
+__global__ void __launch_bounds__(64)
+    powerlaw_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
+                    const double *__restrict__ u, const double *__restrict__ B, double nexp, double eta_min,
+                    double eta_max, double *__restrict__ eta) {
+  __shared__ double sx[kEPB][27][3];
+  __shared__ double su[kEPB][27][3];
+  const int h = threadIdx.x >> 5, t = threadIdx.x & 31;
+  long e = (long)blockIdx.x * kEPB + h;
+  const bool live = e < nel;
+  if (!live) e = nel - 1;
+  if (t < 27) {
+    const long node = elem_node(mx, my, X, XY, e, t);
+    for (int d = 0; d < 3; d++) {
+      sx[h][t][d] = xc[3 * node + d];
+      su[h][t][d] = u[3 * node + d];
+    }
+  }
+  __syncthreads();
+  if (t < 27 && live) {
+    QpGeom g;
+    qp_geometry(sx[h], t, g);
+    double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // L[l][k] = d u_l / d x_k
+    for (int n = 0; n < 27; n++) {
+      double gr[3], gp[3];
+      grad_ref(n, t, gr);
+      for (int k = 0; k < 3; k++) gp[k] = g.G[k][0] * gr[0] + g.G[k][1] * gr[1] + g.G[k][2] * gr[2];
+      for (int l = 0; l < 3; l++)
+        for (int k = 0; k < 3; k++) L[l][k] += su[h][n][l] * gp[k];
+    }
+    double sum = 0.0;
+    for (int l = 0; l < 3; l++)
+      for (int k = 0; k < 3; k++) {
+        const double dlk = 0.5 * (L[l][k] + L[k][l]);
+        sum += dlk * dlk;
+      }
+    double sr = sqrt(0.5 * sum);
+    if (sr < 1.0e-17) sr = 1.0e-17;
+    double v = B[27 * e + t] * pow(sr, 1.0 / nexp - 1.0);
+    v = fmin(fmax(v, eta_min), eta_max);
+    eta[27 * e + t] = v;
+  }
+}
+
 int init_constants() {
   static bool done = false;
   if (done) return PTMF_OK;
@@ -426,6 +474,18 @@ int launch_schur_apply(const ptmf_op_s *op, const double *Minv, const double *rp
   long nb = (n + 255) / 256;
   if (nb > 65536) nb = 65536;
   schur_apply_kernel<<<dim3((unsigned)nb), dim3(256), 0, op->stream>>>(op->box.nel, Minv, rp, yp);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+int launch_powerlaw(const ptmf_op_s *op, const double *u, const double *B, double nexp, double eta_min,
+                    double eta_max, double *eta) {
+  PTMF_TRY(init_constants());
+  const Box &b = op->box;
+  const long nblk = (b.nel + kEPB - 1) / kEPB;
+  powerlaw_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(b.mx, b.my, b.nel, b.X, b.X * b.Y,
+                                                                     op->d64.coords, u, B, nexp, eta_min,
+                                                                     eta_max, eta);
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }

@@ -144,6 +144,9 @@ int launch_bc_copy(const long *idx, long n, const T *x, T *y, hipStream_t s);
 template <typename T>
 int launch_bc_set(const long *idx, long n, T v, T *y, hipStream_t s);
 int launch_copy16(const void *x, void *y, long n16, hipStream_t s);
+int launch_eta_qpmajor(long nel, const double *em, double *q64, float *q32, hipStream_t s);
+int launch_powerlaw(const ptmf_op_s *op, const double *u, const double *B, double nexp, double eta_min,
+                    double eta_max, double *eta);
 int launch_a11_diag(const ptmf_op_s *op, double *d);
 int launch_a11_l1(const ptmf_op_s *op, double *d);
 int launch_schur_blocks(const ptmf_op_s *op, double *Me, double *Minv);

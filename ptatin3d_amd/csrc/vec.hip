@@ -34,6 +34,17 @@ __global__ void __launch_bounds__(256) copy16_kernel(const uint4 *__restrict__ x
     y[i] = x[i];
 }
 
+// element-major [27e+q] -> quadrature-point-major [q*nel+e], FP64 and FP32
+__global__ void eta_qpmajor_kernel(long nel, const double *__restrict__ em, double *__restrict__ q64,
+                                   float *__restrict__ q32) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < 27 * nel; t += (long)gridDim.x * blockDim.x) {
+    const long q = t / nel, e = t - q * nel;
+    const double v = em[27 * e + q];
+    q64[t] = v;
+    q32[t] = (float)v;
+  }
+}
+
 inline unsigned grid_for(long n, int bs) {
   long g = (n + bs - 1) / bs;
   if (g > 65536) g = 65536;
@@ -70,6 +81,12 @@ int launch_bc_set(const long *idx, long n, T v, T *y, hipStream_t s) {
 int launch_copy16(const void *x, void *y, long n16, hipStream_t s) {
   if (n16 <= 0) return PTMF_OK;
   copy16_kernel<<<grid_for(n16, 256), 256, 0, s>>>((const uint4 *)x, (uint4 *)y, n16);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+int launch_eta_qpmajor(long nel, const double *em, double *q64, float *q32, hipStream_t s) {
+  eta_qpmajor_kernel<<<grid_for(27 * nel, 256), 256, 0, s>>>(nel, em, q64, q32);
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }

@@ -97,6 +97,15 @@ class MatA11MF:
         """MatGetDiagonal_MFStokes_A11 (stokes_operators.c:1060-1099)."""
         call("ptmf_a11_diagonal", self.handle, d.ptr)
 
+    def powerlaw_viscosity(self, u: DeviceArray, B: DeviceArray, nexp: float, eta_min: float,
+                           eta_max: float, eta: DeviceArray) -> None:
+        """Quadrature-point power-law viscosity of velocity u (element-major 27E)."""
+        call("ptmf_op_powerlaw_viscosity", self.handle, u.ptr, B.ptr, float(nexp), float(eta_min),
+             float(eta_max), eta.ptr)
+
+    def set_viscosity_device(self, eta: DeviceArray) -> None:
+        call("ptmf_op_set_viscosity_device", self.handle, eta.ptr)
+
     def get_l1_diagonal(self, d: DeviceArray) -> None:
         """l1 row sums of the MatMult operator (the l1-Jacobi scaling)."""
         call("ptmf_a11_l1_diagonal", self.handle, d.ptr)

@@ -15,7 +15,11 @@ transposed operand) would fail:
   a12 / a21    adjoint to each other, == numpy brute force, closed forms;
   transfers    reproduce constants and lattice-linear fields, R == P^T;
   chebyshev    == textbook Chebyshev semi-iteration (same polynomial) and
-               obeys the Chebyshev error bound with exact eigenvalue bounds.
+               obeys the Chebyshev error bound with exact eigenvalue bounds;
+  powerlaw_eta affine u on a curved mesh has grad u = G exactly (closed-form
+               eII), u = (x^2, y z, 0) on a box gives eII at the quadrature
+               points' own positions (pins qp order and element order), the
+               1e-17 strain-rate floor and the clipping bounds.
 """
 import numpy as np
 import pytest
@@ -313,3 +317,38 @@ def test_chebyshev_matches_textbook_and_bound():
     q = (np.sqrt(kappa) - 1) / (np.sqrt(kappa) + 1)
     k = its + 1
     assert en(x) <= 2 * q ** k / (1 + q ** (2 * k)) * en(x0) * (1 + 1e-9)
+
+
+def _powerlaw(B, sr, n, lo, hi):
+    return np.clip(B * np.maximum(sr, 1e-17) ** (1.0 / n - 1.0), lo, hi)
+
+
+def test_powerlaw_affine_curved(curved):
+    m, coords, eta, bc, rng = curved
+    G = rng.uniform(-1, 1, (3, 3))
+    u = (coords.reshape(-1, 3) @ G.T).reshape(-1)
+    D = 0.5 * (G + G.T)
+    sr = np.sqrt(0.5 * np.sum(D * D))
+    B = rng.uniform(1, 10, 27 * m[0] * m[1] * m[2])
+    got = O.powerlaw_eta(*m, coords, u, B, 3.0, 1e-6, 1e6)
+    assert rel_inf(got, _powerlaw(B, sr, 3.0, 1e-6, 1e6)) <= 1e-12
+
+
+def test_powerlaw_quadratic_at_qp_positions():
+    m = (3, 2, 4)
+    coords = F.box_coords(*m, hi=(1.3, 0.8, 1.1))
+    c = coords.reshape(-1, 3)
+    u = np.stack([c[:, 0] ** 2, c[:, 1] * c[:, 2], np.zeros(len(c))], axis=1).reshape(-1)
+    qx = F.qp_coordinates(coords, *m).reshape(-1, 3)  # element-major [27e+q]
+    x, y, z = qx[:, 0], qx[:, 1], qx[:, 2]
+    # D = [[2x,0,0],[0,z,y/2],[0,y/2,0]]
+    sr = np.sqrt(0.5 * (4 * x * x + z * z + 2 * (0.5 * y) ** 2))
+    B = np.full(qx.shape[0], 2.5)
+    got = O.powerlaw_eta(*m, coords, u, B, 2.0, 1e-9, 1e9)
+    assert rel_inf(got, _powerlaw(B, sr, 2.0, 1e-9, 1e9)) <= 1e-12
+    # Newtonian limit n = 1 returns B; zero velocity hits the floor and then eta_max
+    assert np.array_equal(O.powerlaw_eta(*m, coords, u, B, 1.0, 1e-9, 1e9), B)
+    z0 = O.powerlaw_eta(*m, coords, np.zeros_like(u), B, 3.0, 1e-3, 1e3)
+    assert np.all(z0 == 1e3)
+    z1 = O.powerlaw_eta(*m, coords, np.zeros_like(u), B, 3.0, 1e-3, 1e40)
+    assert rel_inf(z1, B * 1e-17 ** (-2.0 / 3.0)) <= 1e-14
