@@ -198,12 +198,20 @@ int ptmf_mg_level_op(ptmf_mg mg, int level, ptmf_op *op);
 int ptmf_mg_set_coordinates(ptmf_mg mg, const double *coords);
 /* quadrature-point viscosity of one level (host, 27E, element-major) */
 int ptmf_mg_set_viscosity(ptmf_mg mg, int level, const double *eta);
-/* fill every coarser level from the next finer one.  Modes 0/1/2: each coarse
- * element takes one arithmetic/geometric/harmonic mean of its 8 children's 216
- * quadrature values.  Modes 3/4: each coarse quadrature point takes the
- * arithmetic/geometric mean of the children containing it (sub-element
- * variation kept).  (The reference projects marker viscosity through a Q1
- * nodal field per level, material_point_utils.c:1267-1420.) */
+/* fill every coarser level from the next finer one, on the device.  Modes
+ * 0/1/2: each coarse element takes one arithmetic/geometric/harmonic mean of
+ * its 8 children's 216 quadrature values.  Modes 3/4: each coarse quadrature
+ * point takes the arithmetic/geometric mean of the children containing it
+ * (sub-element variation kept).  Modes 5/6: the reference's coefficient
+ * hierarchy from quadrature-point values
+ * (SwarmUpdateGaussPropertiesLocalL2Projection_Q1_MPntPStokes_Hierarchy,
+ * material_point_utils.c:1138-1265,1267-1420): a nodal field on the finest
+ * Q2 lattice (corner nodes weighted by the Q1 basis at each quadrature point,
+ * other nodes by 1), carried down by scaled restriction (P^T f)/(P^T 1)
+ * (mode 5, -mp_hierarchy_projection_type 0) or injection (mode 6, type 1),
+ * and interpolated to each level's quadrature points from the element
+ * corners, clamped to the element's nodal range.  The finest level keeps its
+ * own values. */
 int ptmf_mg_coarsen_viscosity(ptmf_mg mg, int mode);
 /* Dirichlet mask of one level (host, 3N bytes) */
 int ptmf_mg_set_dirichlet(ptmf_mg mg, int level, const unsigned char *mask);

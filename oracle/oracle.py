@@ -141,3 +141,36 @@ def powerlaw_eta(mx, my, mz, coords, u, B, nexp, eta_min, eta_max):
     load().orc_powerlaw_eta(mx, my, mz, _p(_f64(coords)), _p(_f64(u)), _p(_f64(B)), C.c_double(nexp),
                             C.c_double(eta_min), C.c_double(eta_max), _p(eta))
     return eta
+
+
+def q1_nodal_from_qp(mx, my, mz, eta):
+    """Fine nodal field of the coefficient hierarchy from element-major qp values."""
+    out = np.zeros(nn(mx, my, mz))
+    load().orc_q1_nodal_from_qp(mx, my, mz, _p(_f64(eta)), _p(out))
+    return out
+
+
+def q1_restrict_scaled(mxc, myc, mzc, nf, inject=False):
+    out = np.zeros(nn(mxc, myc, mzc))
+    load().orc_q1_restrict_scaled(mxc, myc, mzc, int(bool(inject)), _p(_f64(nf)), _p(out))
+    return out
+
+
+def q1_to_qp(mx, my, mz, nodal):
+    out = np.zeros(27 * mx * my * mz)
+    load().orc_q1_to_qp(mx, my, mz, _p(_f64(nodal)), _p(out))
+    return out
+
+
+def q1_projection_hierarchy(dims, eta_fine, inject=False):
+    """Per-level element-major qp viscosity (level 0 coarsest) as the
+    reference's coefficient hierarchy builds it from fine qp values; the
+    finest level keeps its own values."""
+    L = len(dims)
+    out = [None] * L
+    out[L - 1] = np.asarray(eta_fine, dtype=np.float64).copy()
+    nodal = q1_nodal_from_qp(*dims[L - 1], eta_fine)
+    for l in range(L - 1, 0, -1):
+        nodal = q1_restrict_scaled(*dims[l - 1], nodal, inject)
+        out[l - 1] = q1_to_qp(*dims[l - 1], nodal)
+    return out

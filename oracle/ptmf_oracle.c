@@ -757,3 +757,126 @@ void orc_powerlaw_eta(int mx, int my, int mz, const double *coords, const double
   }
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* coefficient hierarchy: Q1 projection of quadrature-point viscosity        */
+/* ------------------------------------------------------------------------ */
+
+/* weight of the qp at reference point xi for local node n of a Q2 element in
+ * the projection: the Q1 basis of the corner for the 8 corner nodes, 1 for
+ * the 19 others (material_point_utils.c:1190-1210) */
+static void q1proj_weights(const double xi[3], double w[27])
+{
+  for (int n = 0; n < 27; n++) w[n] = 1.0;
+  for (int c = 0; c < 2; c++)
+    for (int b = 0; b < 2; b++)
+      for (int a = 0; a < 2; a++) {
+        const double fa = a ? 0.5 * (1.0 + xi[0]) : 0.5 * (1.0 - xi[0]);
+        const double fb = b ? 0.5 * (1.0 + xi[1]) : 0.5 * (1.0 - xi[1]);
+        const double fc = c ? 0.5 * (1.0 + xi[2]) : 0.5 * (1.0 - xi[2]);
+        w[2 * a + 6 * b + 18 * c] = fa * fb * fc;
+      }
+}
+
+/* Fine-level nodal field from the quadrature-point values (element-major
+ * eta[27e+q]): nodal[n] = sum_{e,q} w_q(n) eta_{e,q} / sum_{e,q} w_q(n)
+ * (_BuildQ1CoefficientProjection_QuadraturePoints_MPntPStokes_FineGrid,
+ * material_point_utils.c:1138-1265; element loop, qp loop, then the
+ * pointwise division). */
+void orc_q1_nodal_from_qp(int mx, int my, int mz, const double *eta, double *nodal)
+{
+  box_t b = mkbox(mx, my, mz);
+  static double W[27], XI[27][3], NI[27][27], GNI[27][3][27];
+  prep_q2_3x3(W, XI, NI, GNI);
+  double *B = (double *)calloc((size_t)b.nnodes, sizeof(double));
+  memset(nodal, 0, sizeof(double) * b.nnodes);
+  for (long e = 0; e < b.nel; e++) {
+    double Ae[27] = {0}, Be[27] = {0};
+    long nid[27];
+    elnodes(&b, e, nid);
+    for (int q = 0; q < 27; q++) {
+      double w[27];
+      q1proj_weights(XI[q], w);
+      for (int n = 0; n < 27; n++) {
+        Ae[n] += w[n] * eta[27 * e + q];
+        Be[n] += w[n];
+      }
+    }
+    for (int n = 0; n < 27; n++) {
+      nodal[nid[n]] += Ae[n];
+      B[nid[n]] += Be[n];
+    }
+  }
+  for (long n = 0; n < b.nnodes; n++) nodal[n] /= B[n];
+  free(B);
+}
+
+/* One level down the hierarchy on the node lattice (scalar DMDA Q1
+ * interpolation P between the coarse box and the fine box 2mxc x 2myc x 2mzc):
+ * inject == 0: nc = (P^T nf) * (1 / (P^T 1)) (MatRestrict and
+ * DMCreateInterpolationScale, material_point_utils.c:1373-1386);
+ * inject != 0: nc = nf at the even fine nodes (DMCreateInjection, 1389-1395). */
+void orc_q1_restrict_scaled(int mxc, int myc, int mzc, int inject, const double *nf, double *nc)
+{
+  box_t bc = mkbox(mxc, myc, mzc), bf = mkbox(2 * mxc, 2 * myc, 2 * mzc);
+  if (inject) {
+    for (long k = 0; k < bc.Z; k++)
+      for (long j = 0; j < bc.Y; j++)
+        for (long i = 0; i < bc.X; i++) nc[i + bc.X * (j + bc.Y * k)] = nf[2 * i + bf.X * (2 * j + bf.Y * 2 * k)];
+    return;
+  }
+  double *s = (double *)calloc((size_t)bc.nnodes, sizeof(double));
+  memset(nc, 0, sizeof(double) * bc.nnodes);
+  for (long K = 0; K < bf.Z; K++)
+    for (long J = 0; J < bf.Y; J++)
+      for (long I = 0; I < bf.X; I++) {
+        long ci[2], cj[2], ck[2];
+        double wi[2], wj[2], wk[2];
+        int ni = lat_weights(I, ci, wi), nj = lat_weights(J, cj, wj), nk = lat_weights(K, ck, wk);
+        const long f = I + bf.X * (J + bf.Y * K);
+        for (int c = 0; c < nk; c++)
+          for (int bb = 0; bb < nj; bb++)
+            for (int a = 0; a < ni; a++) {
+              const long g = ci[a] + bc.X * (cj[bb] + bc.Y * ck[c]);
+              const double w = wi[a] * wj[bb] * wk[c];
+              nc[g] += w * nf[f];
+              s[g] += w;
+            }
+      }
+  for (long g = 0; g < bc.nnodes; g++) nc[g] *= 1.0 / s[g];
+  free(s);
+}
+
+/* Quadrature-point values of a level from its nodal field: Q1 interpolation
+ * from the 8 element corners, clamped to the range of the element's 27 nodal
+ * values (_SwarmUpdateGaussPropertiesLocalL2ProjectionQ1_MPntPStokes_Interpolate
+ * ToQuadratePoints, material_point_utils.c:894-1000). */
+void orc_q1_to_qp(int mx, int my, int mz, const double *nodal, double *eta)
+{
+  box_t b = mkbox(mx, my, mz);
+  static double W[27], XI[27][3], NI[27][27], GNI[27][3][27];
+  prep_q2_3x3(W, XI, NI, GNI);
+  for (long e = 0; e < b.nel; e++) {
+    long nid[27];
+    elnodes(&b, e, nid);
+    double lo = 1.0e32, hi = -1.0e32;
+    for (int n = 0; n < 27; n++) {
+      const double v = nodal[nid[n]];
+      if (v < lo) lo = v;
+      if (v > hi) hi = v;
+    }
+    for (int q = 0; q < 27; q++) {
+      double w[27], v = 0.0;
+      q1proj_weights(XI[q], w);
+      for (int c = 0; c < 2; c++)
+        for (int bb = 0; bb < 2; bb++)
+          for (int a = 0; a < 2; a++) {
+            const int n = 2 * a + 6 * bb + 18 * c;
+            v += w[n] * nodal[nid[n]];
+          }
+      if (v < lo) v = lo;
+      if (v > hi) v = hi;
+      eta[27 * e + q] = v;
+    }
+  }
+}

@@ -99,6 +99,19 @@ void orc_cheb_jacobi(int mx, int my, int mz, const double *coords,
 void orc_powerlaw_eta(int mx, int my, int mz, const double *coords, const double *u,
                       const double *B, double nexp, double eta_min, double eta_max, double *eta);
 
+/* Coefficient hierarchy from quadrature-point viscosity
+ * (SwarmUpdateGaussPropertiesLocalL2Projection_Q1_MPntPStokes_Hierarchy,
+ * material_point_utils.c:1267-1420, quadrature-point projection types):
+ *   orc_q1_nodal_from_qp    fine nodal field (corner nodes Q1-weighted, other
+ *                           nodes weight 1), nodal[(2mx+1)(2my+1)(2mz+1)];
+ *   orc_q1_restrict_scaled  next coarser nodal field, (P^T f) / (P^T 1) or
+ *                           injection (inject != 0);
+ *   orc_q1_to_qp            Q1 interpolation to the 27 qps, clamped to the
+ *                           element's nodal range, element-major eta[27e+q]. */
+void orc_q1_nodal_from_qp(int mx, int my, int mz, const double *eta, double *nodal);
+void orc_q1_restrict_scaled(int mxc, int myc, int mzc, int inject, const double *nf, double *nc);
+void orc_q1_to_qp(int mx, int my, int mz, const double *nodal, double *eta);
+
 #ifdef __cplusplus
 }
 #endif

@@ -375,11 +375,12 @@ __global__ void k_restrict_q2(long Xc, long Yc, long Zc, long Xf, long Yf, long 
 }
 
 // coarse-element viscosity from the 8 x 27 quadrature values of the children
-// (element-major input and output): mode 0 arithmetic, 1 geometric,
+// (quadrature-point-major input ef[q * nef + e], element-major output
+// ec[27 E + q]): mode 0 arithmetic, 1 geometric,
 // 2 harmonic mean, one value per coarse element
 __global__ void k_coarsen_eta(int mxc, int myc, int mzc, int mode, const double *__restrict__ ef,
                               double *__restrict__ ec) {
-  const long nec = (long)mxc * myc * mzc;
+  const long nec = (long)mxc * myc * mzc, nef = 8 * nec;
   const int mxf = 2 * mxc, myf = 2 * myc;
   for (long E = blockIdx.x * (long)blockDim.x + threadIdx.x; E < nec; E += (long)gridDim.x * blockDim.x) {
     const long ei = E % mxc, ej = (E / mxc) % myc, ek = E / ((long)mxc * myc);
@@ -389,7 +390,7 @@ __global__ void k_coarsen_eta(int mxc, int myc, int mzc, int mode, const double 
         for (int a = 0; a < 2; a++) {
           const long ef_e = (2 * ei + a) + (long)mxf * ((2 * ej + b) + (long)myf * (2 * ek + c));
           for (int q = 0; q < 27; q++) {
-            const double v = ef[27 * ef_e + q];
+            const double v = ef[(long)q * nef + ef_e];
             s += mode == 0 ? v : (mode == 1 ? log(v) : 1.0 / v);
           }
         }
@@ -405,7 +406,7 @@ __global__ void k_coarsen_eta(int mxc, int myc, int mzc, int mode, const double 
 // qa = 2, both for the midpoint qa = 1), keeping the sub-element variation
 __global__ void k_coarsen_eta_qp(int mxc, int myc, int mzc, int mode, const double *__restrict__ ef,
                                  double *__restrict__ ec) {
-  const long n = 27L * mxc * myc * mzc;
+  const long n = 27L * mxc * myc * mzc, nef = 8L * mxc * myc * mzc;
   const int mxf = 2 * mxc, myf = 2 * myc;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     const long E = t / 27;
@@ -419,13 +420,138 @@ __global__ void k_coarsen_eta_qp(int mxc, int myc, int mzc, int mode, const doub
         for (int a = (qa == 2 ? 1 : 0); a <= (qa == 0 ? 0 : 1); a++) {
           const long ef_e = (2 * ei + a) + (long)mxf * ((2 * ej + b) + (long)myf * (2 * ek + c));
           for (int r = 0; r < 27; r++) {
-            const double v = ef[27 * ef_e + r];
+            const double v = ef[(long)r * nef + ef_e];
             s += mode == 3 ? v : log(v);
           }
           cnt += 27;
         }
     s /= (double)cnt;
     ec[t] = mode == 3 ? s : exp(s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient hierarchy from quadrature-point viscosity
+// (SwarmUpdateGaussPropertiesLocalL2Projection_Q1_MPntPStokes_Hierarchy,
+// material_point_utils.c:1267-1420, with the fine field built from the
+// quadrature points, 1138-1265): a nodal field on each level's Q2 node
+// lattice, carried down by scaled restriction or injection, interpolated to
+// the level's quadrature points from the 8 element corners and clamped to the
+// element's nodal range (894-1000).
+// ---------------------------------------------------------------------------
+// Q1 factor of corner side s (0: xi = -1, 1: xi = +1) at Gauss abscissa a
+__device__ __forceinline__ double q1_factor(int s, int a) {
+  const double x = a == 0 ? -kS : (a == 1 ? 0.0 : kS);
+  return s ? 0.5 * (1.0 + x) : 0.5 * (1.0 - x);
+}
+
+// projection weight of quadrature point q for local node (i, j, k): the Q1
+// corner basis at the 8 corners, 1 at the other 19 nodes
+__device__ __forceinline__ double q1_proj_weight(int i, int j, int k, int q) {
+  if ((i | j | k) & 1) return 1.0;
+  return q1_factor(i >> 1, q % 3) * q1_factor(j >> 1, (q / 3) % 3) * q1_factor(k >> 1, q / 9);
+}
+
+// per lattice coordinate: the (element, local index) pairs containing it,
+// in increasing element order
+__device__ __forceinline__ int lattice_owners(long I, int m, long *el, int *loc) {
+  if (I & 1) {
+    el[0] = (I - 1) / 2; loc[0] = 1;
+    return 1;
+  }
+  int n = 0;
+  if (I / 2 - 1 >= 0) { el[n] = I / 2 - 1; loc[n] = 2; n++; }
+  if (I / 2 < m) { el[n] = I / 2; loc[n] = 0; n++; }
+  return n;
+}
+
+// fine nodal field from qp-major viscosity eq[q * nel + e]; one thread per
+// node gathers its (at most 8) elements in increasing order, each element's
+// sum over its quadrature points in order (the reference's scatter order)
+__global__ void k_q1_nodal(int mx, int my, int mz, const double *__restrict__ eq, double *__restrict__ nodal) {
+  const long X = 2L * mx + 1, Y = 2L * my + 1, N = X * Y * (2L * mz + 1);
+  const long nel = (long)mx * my * mz;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < N; g += (long)gridDim.x * blockDim.x) {
+    const long I = g % X, J = (g / X) % Y, K = g / (X * Y);
+    long ex[2], ey[2], ez[2];
+    int lx[2], ly[2], lz[2];
+    const int nx = lattice_owners(I, mx, ex, lx), ny = lattice_owners(J, my, ey, ly),
+              nz = lattice_owners(K, mz, ez, lz);
+    double A = 0.0, B = 0.0;
+    for (int c = 0; c < nz; c++)
+      for (int b = 0; b < ny; b++)
+        for (int a = 0; a < nx; a++) {
+          const long e = ex[a] + mx * (ey[b] + (long)my * ez[c]);
+          double Ae = 0.0, Be = 0.0;
+          for (int q = 0; q < 27; q++) {
+            const double w = q1_proj_weight(lx[a], ly[b], lz[c], q);
+            Ae += w * eq[(long)q * nel + e];
+            Be += w;
+          }
+          A += Ae;
+          B += Be;
+        }
+    nodal[g] = A / B;
+  }
+}
+
+// next coarser nodal field: (P^T f) * (1 / (P^T 1)) with the scalar DMDA Q1
+// lattice interpolation P, or injection
+__global__ void k_q1_restrict(int mxc, int myc, int mzc, int inject, const double *__restrict__ nf,
+                              double *__restrict__ nc) {
+  const long Xc = 2L * mxc + 1, Yc = 2L * myc + 1, Nc = Xc * Yc * (2L * mzc + 1);
+  const long Xf = 4L * mxc + 1, Yf = 4L * myc + 1, Zf = 4L * mzc + 1;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < Nc; g += (long)gridDim.x * blockDim.x) {
+    const long I = g % Xc, J = (g / Xc) % Yc, K = g / (Xc * Yc);
+    if (inject) {
+      nc[g] = nf[2 * I + Xf * (2 * J + Yf * 2 * K)];
+      continue;
+    }
+    double a = 0.0, s = 0.0;
+    for (int dk = -1; dk <= 1; dk++) {
+      const long Kf = 2 * K + dk;
+      if (Kf < 0 || Kf >= Zf) continue;
+      for (int dj = -1; dj <= 1; dj++) {
+        const long Jf = 2 * J + dj;
+        if (Jf < 0 || Jf >= Yf) continue;
+        for (int di = -1; di <= 1; di++) {
+          const long If = 2 * I + di;
+          if (If < 0 || If >= Xf) continue;
+          const double w = (di ? 0.5 : 1.0) * (dj ? 0.5 : 1.0) * (dk ? 0.5 : 1.0);
+          a += w * nf[If + Xf * (Jf + Yf * Kf)];
+          s += w;
+        }
+      }
+    }
+    nc[g] = a * (1.0 / s);
+  }
+}
+
+// element-major quadrature values of a level from its nodal field: Q1
+// interpolation from the element corners, clamped to the range of the 27
+// element nodes
+__global__ void k_q1_to_qp(int mx, int my, int mz, const double *__restrict__ nodal, double *__restrict__ ec) {
+  const long X = 2L * mx + 1, XY = X * (2L * my + 1);
+  const long n = 27L * mx * my * mz;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const long e = t / 27;
+    const int q = (int)(t - 27 * e);
+    const long ei = e % mx, ej = (e / mx) % my, ek = e / ((long)mx * my);
+    const long base = 2 * ei + X * (2 * ej) + XY * (2 * ek);
+    double lo = 1.0e32, hi = -1.0e32;
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+          const double v = nodal[base + i + X * j + XY * k];
+          lo = fmin(lo, v);
+          hi = fmax(hi, v);
+        }
+    double v = 0.0;
+    for (int c = 0; c < 2; c++)
+      for (int b = 0; b < 2; b++)
+        for (int a = 0; a < 2; a++)
+          v += q1_proj_weight(2 * a, 2 * b, 2 * c, q) * nodal[base + 2 * a + X * 2 * b + XY * 2 * c];
+    ec[t] = fmin(fmax(v, lo), hi);
   }
 }
 
@@ -1345,36 +1471,59 @@ int ptmf_mg_set_viscosity(ptmf_mg mg, int level, const double *eta) {
 
 int ptmf_mg_coarsen_viscosity(ptmf_mg mg, int mode) {
   PTMF_CHECK(mg, PTMF_ERR_ARG, "mg_coarsen_viscosity: null");
-  PTMF_CHECK(mode >= 0 && mode <= 4, PTMF_ERR_UNSUPPORTED,
+  PTMF_CHECK(mode >= 0 && mode <= 6, PTMF_ERR_UNSUPPORTED,
              "mg_coarsen_viscosity: mode %d (0 arithmetic, 1 geometric, 2 harmonic element means; "
-             "3 arithmetic, 4 geometric quadrature-point-local means)", mode);
+             "3 arithmetic, 4 geometric quadrature-point-local means; 5 Q1 projection with scaled "
+             "restriction, 6 with injection)", mode);
   PTMF_CHECK(mg->lev[mg->L - 1].have_eta, PTMF_ERR_NOT_SETUP, "mg_coarsen_viscosity: finest viscosity not set");
-  for (int l = mg->L - 1; l > 0; l--) {
-    MGLevel &F = mg->lev[l], &Cl = mg->lev[l - 1];
-    const long nef = F.op->box.nel, nec = Cl.op->box.nel;
-    double *ef = nullptr, *ec = nullptr;
-    PTMF_TRY(ptmf::dalloc((void **)&ef, sizeof(double) * 27 * nef));
-    PTMF_TRY(ptmf::dalloc((void **)&ec, sizeof(double) * 27 * nec));
-    // the operator stores eta qp-major; rebuild element-major on the host
-    std::vector<double> hq(27 * nef), he(27 * nef);
-    PTMF_HIP(hipMemcpy(hq.data(), F.op->d64.eta, sizeof(double) * 27 * nef, hipMemcpyDeviceToHost));
-    for (long e = 0; e < nef; e++)
-      for (int q = 0; q < 27; q++) he[27 * e + q] = hq[(long)q * nef + e];
-    PTMF_HIP(hipMemcpy(ef, he.data(), sizeof(double) * 27 * nef, hipMemcpyHostToDevice));
-    if (mode <= 2)
-      ptmf::k_coarsen_eta<<<ptmf::nblocks(nec), ptmf::kBS, 0, mg->stream>>>(Cl.mx, Cl.my, Cl.mz, mode, ef, ec);
-    else
-      ptmf::k_coarsen_eta_qp<<<ptmf::nblocks(27 * nec), ptmf::kBS, 0, mg->stream>>>(Cl.mx, Cl.my, Cl.mz, mode, ef,
-                                                                                  ec);
-    PTMF_HIP(hipGetLastError());
-    std::vector<double> hc(27 * nec);
-    PTMF_HIP(hipMemcpyAsync(hc.data(), ec, sizeof(double) * 27 * nec, hipMemcpyDeviceToHost, mg->stream));
-    PTMF_HIP(hipStreamSynchronize(mg->stream));
-    (void)hipFree(ef);
-    (void)hipFree(ec);
-    PTMF_TRY(ptmf_op_set_viscosity(Cl.op, hc.data()));
-    Cl.have_eta = true;
+  // device scratch: coarse element-major values and, for the projection, two
+  // nodal fields (fine and next coarser)
+  const ptmf::MGLevel &Fin = mg->lev[mg->L - 1];
+  const long nnf = (long)(2 * Fin.mx + 1) * (2 * Fin.my + 1) * (2 * Fin.mz + 1);
+  double *ec = nullptr, *nod[2] = {nullptr, nullptr};
+  int st = ptmf::dalloc((void **)&ec, sizeof(double) * 27 * (Fin.op->box.nel / 8 + 1));
+  if (st == PTMF_OK && mode >= 5) st = ptmf::dalloc((void **)&nod[0], sizeof(double) * nnf);
+  if (st == PTMF_OK && mode >= 5) st = ptmf::dalloc((void **)&nod[1], sizeof(double) * nnf);
+  hipStream_t s = mg->stream;
+  if (st == PTMF_OK && mode >= 5) {
+    ptmf::k_q1_nodal<<<ptmf::nblocks(nnf), ptmf::kBS, 0, s>>>(Fin.mx, Fin.my, Fin.mz, Fin.op->d64.eta, nod[0]);
+    if (hipGetLastError() != hipSuccess) st = ptmf_set_error(PTMF_ERR_DEVICE, "mg_coarsen_viscosity: launch");
   }
+  int cur = 0;
+  for (int l = mg->L - 1; l > 0 && st == PTMF_OK; l--) {
+    ptmf::MGLevel &F = mg->lev[l], &Cl = mg->lev[l - 1];
+    const long nec = Cl.op->box.nel;
+    if (mode <= 2) {
+      ptmf::k_coarsen_eta<<<ptmf::nblocks(nec), ptmf::kBS, 0, s>>>(Cl.mx, Cl.my, Cl.mz, mode, F.op->d64.eta, ec);
+    } else if (mode <= 4) {
+      ptmf::k_coarsen_eta_qp<<<ptmf::nblocks(27 * nec), ptmf::kBS, 0, s>>>(Cl.mx, Cl.my, Cl.mz, mode,
+                                                                         F.op->d64.eta, ec);
+    } else {
+      const long nnc = (long)(2 * Cl.mx + 1) * (2 * Cl.my + 1) * (2 * Cl.mz + 1);
+      ptmf::k_q1_restrict<<<ptmf::nblocks(nnc), ptmf::kBS, 0, s>>>(Cl.mx, Cl.my, Cl.mz, mode == 6, nod[cur],
+                                                                  nod[1 - cur]);
+      cur = 1 - cur;
+      ptmf::k_q1_to_qp<<<ptmf::nblocks(27 * nec), ptmf::kBS, 0, s>>>(Cl.mx, Cl.my, Cl.mz, nod[cur], ec);
+    }
+    if (hipGetLastError() != hipSuccess) {
+      st = ptmf_set_error(PTMF_ERR_DEVICE, "mg_coarsen_viscosity: launch");
+      break;
+    }
+    // the coarse operator's qp-major FP64 and FP32 viscosity, written on the device
+    st = ptmf::launch_eta_qpmajor(nec, ec, Cl.op->d64.eta, Cl.op->d32.eta, s);
+    if (st == PTMF_OK && hipStreamSynchronize(s) != hipSuccess)
+      st = ptmf_set_error(PTMF_ERR_DEVICE, "mg_coarsen_viscosity: synchronize");
+    if (st == PTMF_OK) {
+      Cl.op->has_eta = 1;
+      Cl.op->state++;
+      Cl.have_eta = true;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(ec);
+  (void)hipFree(nod[0]);
+  (void)hipFree(nod[1]);
+  if (st != PTMF_OK) return st;
   mg->is_setup = false;
   return PTMF_OK;
 }

@@ -63,7 +63,8 @@ class GMG:
         call("ptmf_mg_set_viscosity", self.handle, int(level), _lib._hp(e))
 
     def coarsen_viscosity(self, mode: str = "arithmetic") -> None:
-        modes = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "qp-arithmetic": 3, "qp-geometric": 4}
+        modes = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "qp-arithmetic": 3, "qp-geometric": 4,
+                 "q1-projection": 5, "q1-injection": 6}
         call("ptmf_mg_coarsen_viscosity", self.handle, modes[mode])
 
     def set_dirichlet(self, level: int, mask: np.ndarray) -> None:

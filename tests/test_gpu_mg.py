@@ -255,3 +255,57 @@ def test_vcycle_graph_replays_match_eager(gpu, prec):
         _ = np.linalg.norm(ye - b)  # eager host work between replays
     mg_e.destroy()
     mg_g.destroy()
+
+
+def _coarsen_numpy(dims_c, ef, mode):
+    """Element / quadrature-point-local means of the children (modes 0-4)."""
+    mxc, myc, mzc = dims_c
+    e = ef.reshape(2 * mzc, 2 * myc, 2 * mxc, 3, 3, 3)  # [ek, ej, ei, qc, qb, qa]
+    f = {0: lambda v: v, 1: np.log, 2: lambda v: 1 / v, 3: lambda v: v, 4: np.log}[mode]
+    g = {0: lambda v: v, 1: np.exp, 2: lambda v: 1 / v, 3: lambda v: v, 4: np.exp}[mode]
+    c = f(e).reshape(mzc, 2, myc, 2, mxc, 2, 27)
+    if mode <= 2:
+        m = g(c.mean(axis=(1, 3, 5, 6)))
+        return np.repeat(m.reshape(-1), 27)
+    out = np.zeros((mzc, myc, mxc, 3, 3, 3))
+    kids = {0: [0], 1: [0, 1], 2: [1]}
+    for qc in range(3):
+        for qb in range(3):
+            for qa in range(3):
+                sel = c[:, kids[qc]][:, :, :, kids[qb]][:, :, :, :, :, kids[qa]]
+                out[:, :, :, qc, qb, qa] = g(sel.mean(axis=(1, 3, 5, 6)))
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("mode", ["arithmetic", "geometric", "harmonic", "qp-arithmetic", "qp-geometric",
+                                  "q1-projection", "q1-injection"])
+def test_coarsen_viscosity_levels(gpu, mode):
+    """Every coarse level's operator equals the oracle operator with the
+    viscosity the mode defines (Q1 modes: the oracle's coefficient hierarchy)."""
+    m, L = (8, 4, 12), 3
+    rng = np.random.default_rng(17)
+    mg = GMG(*m, L)
+    coords = F.perturb_interior(F.box_coords(*m), *m, 0.1, rng)
+    mg.set_coordinates(coords)
+    eta = F.eta_loguniform(m[0] * m[1] * m[2], 1e-3, 1e3, rng)
+    mg.set_viscosity(L - 1, eta)
+    mg.coarsen_viscosity(mode)
+    mg.set_dirichlet_all_faces()
+    mg.setup(default_options())
+    if mode.startswith("q1"):
+        etas = O.q1_projection_hierarchy(mg.dims, eta, inject=(mode == "q1-injection"))
+    else:
+        code = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "qp-arithmetic": 3, "qp-geometric": 4}[mode]
+        etas = [None] * L
+        etas[L - 1] = eta
+        for l in range(L - 1, 0, -1):
+            etas[l - 1] = _coarsen_numpy(mg.dims[l - 1], etas[l], code)
+    cl = mg.level_coordinates(coords)
+    for l in range(L - 1):
+        dims = mg.dims[l]
+        x = rng.uniform(-1, 1, 3 * F.nnodes(*dims))
+        dx, dy = DeviceArray.from_host(x), DeviceArray(x.size)
+        mg.level_apply(l, dx, dy)
+        ref = O.a11_matmult(*dims, cl[l], etas[l], F.dirichlet_all_faces(*dims), x)
+        assert rel_inf(dy.download(), ref) <= 1e-12, (mode, l)
+    mg.destroy()

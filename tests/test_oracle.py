@@ -19,7 +19,11 @@ transposed operand) would fail:
   powerlaw_eta affine u on a curved mesh has grad u = G exactly (closed-form
                eII), u = (x^2, y z, 0) on a box gives eII at the quadrature
                points' own positions (pins qp order and element order), the
-               1e-17 strain-rate floor and the clipping bounds.
+               1e-17 strain-rate floor and the clipping bounds;
+  q1 hierarchy == numpy restatement with explicit weight matrices (the
+               projection weights, R = P^T from the pinned restrict_add, the
+               clamped Q1 interpolation), constants reproduced exactly,
+               values stay inside the fine range.
 """
 import numpy as np
 import pytest
@@ -352,3 +356,53 @@ def test_powerlaw_quadratic_at_qp_positions():
     assert np.all(z0 == 1e3)
     z1 = O.powerlaw_eta(*m, coords, np.zeros_like(u), B, 3.0, 1e-3, 1e40)
     assert rel_inf(z1, B * 1e-17 ** (-2.0 / 3.0)) <= 1e-14
+
+
+def _q1_hierarchy_numpy(dims, eta, inject=False):
+    """Independent restatement: explicit per-element weight tables."""
+    g = F.GAUSS
+    f1 = np.stack([0.5 * (1 - g), 0.5 * (1 + g)], axis=1)  # [qp 1-D, corner a]
+    wq1 = np.einsum("ia,jb,kc->kjicba", f1, f1, f1).reshape(27, 8)  # [q, corner]
+    corners = np.array([0, 2, 6, 8, 18, 20, 24, 26])  # corner (a,b,c) -> local node 2a+6b+18c
+    Wt = np.ones((27, 27))  # [q, local node]
+    Wt[:, corners] = wq1
+    mx, my, mz = dims[-1]
+    nid = F.element_node_ids(mx, my, mz)
+    e27 = eta.reshape(-1, 27)
+    A = np.zeros(F.nnodes(mx, my, mz))
+    Bw = np.zeros_like(A)
+    np.add.at(A, nid, e27 @ Wt)
+    np.add.at(Bw, nid, np.broadcast_to(Wt.sum(axis=0), nid.shape))
+    nodal = A / Bw
+    out = [None] * len(dims)
+    out[-1] = eta
+    for l in range(len(dims) - 1, 0, -1):
+        cm = dims[l - 1]
+        if inject:
+            X, Y, Z = (2 * dims[l][0] + 1, 2 * dims[l][1] + 1, 2 * dims[l][2] + 1)
+            nodal = nodal.reshape(Z, Y, X)[::2, ::2, ::2].reshape(-1)
+        else:
+            v3 = np.stack([nodal, np.ones_like(nodal), np.zeros_like(nodal)], axis=1).reshape(-1)
+            r = O.restrict_add(*cm, v3, np.zeros(3 * F.nnodes(*cm))).reshape(-1, 3)
+            nodal = r[:, 0] / r[:, 1]
+        nidc = F.element_node_ids(*cm)
+        ev = nodal[nidc]  # [E, 27]
+        q = ev[:, corners] @ wq1.T  # [E, 27]
+        q = np.clip(q, ev.min(axis=1, keepdims=True), ev.max(axis=1, keepdims=True))
+        out[l - 1] = q.reshape(-1)
+    return out
+
+
+@pytest.mark.parametrize("inject", [False, True])
+def test_q1_hierarchy_against_numpy(inject):
+    dims = [(2, 1, 3), (4, 2, 6), (8, 4, 12)]
+    rng = np.random.default_rng(3)
+    eta = F.eta_loguniform(8 * 4 * 12, 1e-3, 1e3, rng)
+    got = O.q1_projection_hierarchy(dims, eta, inject)
+    ref = _q1_hierarchy_numpy(dims, eta, inject)
+    for g, r in zip(got, ref):
+        assert rel_inf(g, r) <= 1e-13
+        assert g.min() >= eta.min() * (1 - 1e-14) and g.max() <= eta.max() * (1 + 1e-14)
+    const = O.q1_projection_hierarchy(dims, np.full_like(eta, 3.25), inject)
+    for g in const:
+        assert np.abs(g - 3.25).max() <= 1e-14
