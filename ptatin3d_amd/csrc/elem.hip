@@ -27,17 +27,37 @@ struct QpGeom {
 // Geometry at quadrature point q = (qa,qb,qc) from the element's node
 // coordinates xe[n][d] (P3D_evaluate_geometry_elementQ2,
 // element_utils_q2.c:400-476, with the exact 3-point Gauss rule).
+// value of a compile-time table column chosen by a lane-dependent index
+__device__ __forceinline__ double pick3(int i, double v0, double v1, double v2) {
+  return i == 0 ? v0 : (i == 1 ? v1 : v2);
+}
+
 __device__ void qp_geometry(const double (*xe)[3], int q, QpGeom &g) {
   const int qa = q % 3, qb = (q / 3) % 3, qc = q / 9;
+  // the lane's rows of the 1-D tables, held in registers
+  double Ba[3], Da[3], Bb[3], Db[3], Bc[3], Dc[3];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    Ba[r] = pick3(qa, kB[0][r], kB[1][r], kB[2][r]);
+    Da[r] = pick3(qa, kD[0][r], kD[1][r], kD[2][r]);
+    Bb[r] = pick3(qb, kB[0][r], kB[1][r], kB[2][r]);
+    Db[r] = pick3(qb, kD[0][r], kD[1][r], kD[2][r]);
+    Bc[r] = pick3(qc, kB[0][r], kB[1][r], kB[2][r]);
+    Dc[r] = pick3(qc, kD[0][r], kD[1][r], kD[2][r]);
+  }
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   double xq[3] = {0, 0, 0};
+#pragma unroll
   for (int k = 0; k < 3; k++)
+#pragma unroll
     for (int j = 0; j < 3; j++)
+#pragma unroll
       for (int i = 0; i < 3; i++) {
         const int n = i + 3 * j + 9 * k;
-        const double Ni = cB[qa][i], Nj = cB[qb][j], Nk = cB[qc][k];
-        const double d0 = cD[qa][i] * Nj * Nk, d1 = Ni * cD[qb][j] * Nk, d2 = Ni * Nj * cD[qc][k];
+        const double Ni = Ba[i], Nj = Bb[j], Nk = Bc[k];
+        const double d0 = Da[i] * Nj * Nk, d1 = Ni * Db[j] * Nk, d2 = Ni * Nj * Dc[k];
         const double N = Ni * Nj * Nk;
+#pragma unroll
         for (int d = 0; d < 3; d++) {
           const double x = xe[n][d];
           J[0][d] += d0 * x;
@@ -60,7 +80,7 @@ __device__ void qp_geometry(const double (*xe)[3], int q, QpGeom &g) {
   const double id = 1.0 / det;
   for (int r = 0; r < 3; r++)
     for (int s = 0; s < 3; s++) g.G[r][s] = A[r][s] * id;
-  g.wdet = cW[qa] * cW[qb] * cW[qc] * det;
+  g.wdet = pick3(qa, kW[0], kW[1], kW[2]) * pick3(qb, kW[0], kW[1], kW[2]) * pick3(qc, kW[0], kW[1], kW[2]) * det;
   for (int d = 0; d < 3; d++) g.xq[d] = xq[d];
 }
 
@@ -94,14 +114,19 @@ __device__ __forceinline__ void p1_basis(const double (*xe)[3], const double xq[
 // (MFStokesWrapper_diagA11, stokes_operators_mf.c:67-131 with
 // MatMultMF_Stokes_MixedFEM3d_diagB11, stokes_q2p1_mf_operators_diag_def.c:120-205)
 __global__ void __launch_bounds__(64)
-    a11_diag_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
+    a11_diag_kernel(int mx, int my, int mz, long nel, long X, long XY, int color, const double *__restrict__ xc,
                     const double *__restrict__ eta, double *__restrict__ dg) {
   __shared__ double sx[kEPB][27][3];
   __shared__ double sG[kEPB][27][10];
   const int h = threadIdx.x >> 5, t = threadIdx.x & 31;
-  long e = (long)blockIdx.x * kEPB + h;
-  const bool live = e < nel;
-  if (!live) e = nel - 1;
+  // elements of one parity class (color = px + 2 py + 4 pz): no two of them
+  // share a node, so their sums go in with plain read-add-write
+  const int px = color & 1, py = (color >> 1) & 1, pz = color >> 2;
+  const long ncx = (mx - px + 1) / 2, ncy = (my - py + 1) / 2, ncz = (mz - pz + 1) / 2;
+  long slot = (long)blockIdx.x * kEPB + h;
+  const bool live = slot < ncx * ncy * ncz;
+  if (!live) slot = ncx * ncy * ncz - 1;
+  const long e = (2 * (slot % ncx) + px) + mx * ((2 * ((slot / ncx) % ncy) + py) + (long)my * (2 * (slot / (ncx * ncy)) + pz));
   if (t < 27) {
     const long node = elem_node(mx, my, X, XY, e, t);
     for (int d = 0; d < 3; d++) sx[h][t][d] = xc[3 * node + d];
@@ -116,18 +141,41 @@ __global__ void __launch_bounds__(64)
   }
   __syncthreads();
   if (t < 27 && live) {
+    // columns (i, j, k) = node t of the 1-D tables, in registers
+    const int ni = t % 3, nj = (t / 3) % 3, nk = t / 9;
+    double Bi[3], Di[3], Bj[3], Dj[3], Bk[3], Dk[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      Bi[a] = pick3(ni, kB[a][0], kB[a][1], kB[a][2]);
+      Di[a] = pick3(ni, kD[a][0], kD[a][1], kD[a][2]);
+      Bj[a] = pick3(nj, kB[a][0], kB[a][1], kB[a][2]);
+      Dj[a] = pick3(nj, kD[a][0], kD[a][1], kD[a][2]);
+      Bk[a] = pick3(nk, kB[a][0], kB[a][1], kB[a][2]);
+      Dk[a] = pick3(nk, kD[a][0], kD[a][1], kD[a][2]);
+    }
     double acc[3] = {0, 0, 0};
-    for (int q = 0; q < 27; q++) {
-      double gr[3], gp[3];
-      grad_ref(t, q, gr);
-      const double *G = sG[h][q];
-      for (int k = 0; k < 3; k++) gp[k] = G[3 * k] * gr[0] + G[3 * k + 1] * gr[1] + G[3 * k + 2] * gr[2];
-      const double s = G[9];
-      const double g2 = gp[0] * gp[0] + gp[1] * gp[1] + gp[2] * gp[2];
-      for (int c = 0; c < 3; c++) acc[c] += s * (g2 + gp[c] * gp[c]);
+    // one quadrature plane per trip (the unrolled 9 points keep their
+    // geometry loads in registers; all 27 at once would spill)
+#pragma unroll 1
+    for (int qc = 0; qc < 3; qc++) {
+      const double bk = pick3(qc, Bk[0], Bk[1], Bk[2]), dk = pick3(qc, Dk[0], Dk[1], Dk[2]);
+#pragma unroll
+      for (int qb = 0; qb < 3; qb++)
+#pragma unroll
+        for (int qa = 0; qa < 3; qa++) {
+          const double *G = sG[h][qa + 3 * qb + 9 * qc];
+          const double gr[3] = {Di[qa] * Bj[qb] * bk, Bi[qa] * Dj[qb] * bk, Bi[qa] * Bj[qb] * dk};
+          double gp[3];
+#pragma unroll
+          for (int k = 0; k < 3; k++) gp[k] = G[3 * k] * gr[0] + G[3 * k + 1] * gr[1] + G[3 * k + 2] * gr[2];
+          const double s = G[9];
+          const double g2 = gp[0] * gp[0] + gp[1] * gp[1] + gp[2] * gp[2];
+#pragma unroll
+          for (int c = 0; c < 3; c++) acc[c] += s * (g2 + gp[c] * gp[c]);
+        }
     }
     const long node = elem_node(mx, my, X, XY, e, t);
-    for (int c = 0; c < 3; c++) atomicAdd(&dg[3 * node + c], acc[c]);
+    for (int c = 0; c < 3; c++) dg[3 * node + c] += acc[c];
   }
 }
 
@@ -442,10 +490,15 @@ int init_constants() {
 int launch_a11_diag(const ptmf_op_s *op, double *d) {
   PTMF_TRY(init_constants());
   const Box &b = op->box;
-  const long nblk = (b.nel + kEPB - 1) / kEPB;
-  a11_diag_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
-      b.mx, b.my, b.nel, b.X, b.X * b.Y, op->d64.coords, op->d64.eta, d);
-  PTMF_HIP(hipGetLastError());
+  for (int color = 0; color < 8; color++) {
+    const long ncol = (long)((b.mx - (color & 1) + 1) / 2) * ((b.my - ((color >> 1) & 1) + 1) / 2) *
+                      ((b.mz - (color >> 2) + 1) / 2);
+    if (ncol == 0) continue;
+    const long nblk = (ncol + kEPB - 1) / kEPB;
+    a11_diag_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
+        b.mx, b.my, b.mz, b.nel, b.X, b.X * b.Y, color, op->d64.coords, op->d64.eta, d);
+    PTMF_HIP(hipGetLastError());
+  }
   return PTMF_OK;
 }
 
