@@ -168,6 +168,12 @@ int ptmf_a11_l1_diagonal(ptmf_op op, double *d);
 int ptmf_a12_mult(ptmf_op op, const double *p, double *yu);
 /* MatMult_MFStokes_A21: yp = B^T mask(u) (yp: 4E). */
 int ptmf_a21_mult(ptmf_op op, const double *u, double *yp);
+/* MatMult_MFStokes_A (stokes_operators.c:884-967, the fused momentum +
+ * continuity wrapper MFStokesWrapper_A): x = [u (3N) | p (4E)], y = K x with
+ * K = [A11 A12; A21 0], Dirichlet velocity columns ignored and Dirichlet
+ * velocity rows y_u = x_u.  One pass over the elements (the pressure enters
+ * the A11 stress at each quadrature point). */
+int ptmf_stokes_mult(ptmf_op op, const double *x, double *y);
 /* Me[16e + 4i + j] = -sum_q w detJ / eta_q NIp_i NIp_j: the element blocks of
  * the scaled viscous pressure mass matrix used as Schur complement
  * approximation (MatAssemble_StokesPC_ScaledMassMatrix, stokes_assembly.c:299-450) */

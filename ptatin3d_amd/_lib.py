@@ -63,6 +63,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_a11_l1_diagonal": (_I, [_P, _P]),
     "ptmf_a12_mult": (_I, [_P, _P, _P]),
     "ptmf_a21_mult": (_I, [_P, _P, _P]),
+    "ptmf_stokes_mult": (_I, [_P, _P, _P]),
     "ptmf_a11_time": (_I, [_P, _I, _P, _P, _I, _PD]),
     "ptmf_mg_create": (_I, [_I, _I, _I, _I, C.POINTER(_P)]),
     "ptmf_mg_destroy": (_I, [C.POINTER(_P)]),

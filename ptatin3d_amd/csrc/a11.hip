@@ -220,10 +220,22 @@ __device__ __forceinline__ void plane_forward(int cp, int le, bool writer, const
 // A work-group is a single wave, so the LDS operations of all lanes execute
 // in program order: a slot read by every lane at qp p is overwritten only
 // after those reads.
+// Per-lane data of the pressure coupling in the fused saddle-point MatMult:
+// the element centroid xb, the pressure coefficients with the element
+// lengths divided out, p_h(x) = pc0 + sum_d pc[1+d] (x_d - xb_d) (the P1GRel
+// basis, element_utils_q2.c:183-224), and the lane's partial sums of
+// B^T u against {1, x - xb, y - yb, z - zb}.
 template <typename T>
+struct PCoupling {
+  T xb[3];
+  T pc[4];
+  T acc[4];
+};
+
+template <typename T, bool kStokes = false>
 __device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq[3][9],
                                          const T Uq[3][9], const T etq9[9],
-                                         T (*sh)[kSlots][kEPW]) {
+                                         T (*sh)[kSlots][kEPW], PCoupling<T> *pcp = nullptr) {
   T dz[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) dz[k] = sel3<T>(cp, kD[0][k], kD[1][k], kD[2][k]);
@@ -284,6 +296,20 @@ __device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq
       sg[0][1] = sg[1][0] = hs * (gh[0][1] + gh[1][0]);
       sg[0][2] = sg[2][0] = hs * (gh[0][2] + gh[2][0]);
       sg[1][2] = sg[2][1] = hs * (gh[1][2] + gh[2][1]);
+      if constexpr (kStokes) {
+        // A12: the pressure enters the stress as -p_h I; A21: the lane's
+        // share of -sum_q w detJ NIp_i div u (w detJ div u = w tr(gh))
+        const T d0 = Xq[0][p] - pcp->xb[0], d1 = Xq[1][p] - pcp->xb[1], d2 = Xq[2][p] - pcp->xb[2];
+        const T wp = wq * (pcp->pc[0] + pcp->pc[1] * d0 + pcp->pc[2] * d1 + pcp->pc[3] * d2);
+        sg[0][0] -= wp;
+        sg[1][1] -= wp;
+        sg[2][2] -= wp;
+        const T dv = -wq * (gh[0][0] + gh[1][1] + gh[2][2]);
+        pcp->acc[0] += dv;
+        pcp->acc[1] += dv * d0;
+        pcp->acc[2] += dv * d1;
+        pcp->acc[3] += dv * d2;
+      }
       // every lane has read slot p of all planes; reuse it for the zeta data
       __builtin_amdgcn_wave_barrier();
       // dv[j][l] = w detJ sum_m sigma[m][l] d xi_j / d x_m
@@ -356,6 +382,70 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
   plane_forward<T, kMask>(cp, le, writer, r, sh, Xq, Uq);
   plane_qp<T>(cp, le, writer, Xq, Uq, r.eta, sh);
   plane_backward<T>(cp, le, sh, out);
+}
+
+// The same element computation for the saddle-point operator
+// [A11 A12; A21 0] (MatMult_MFStokes_A, stokes_operators.c:884-967): the
+// element's pressure pe[0..3] enters the stress, and B^T of the masked
+// velocity is written to ype[0..3] by the lane of plane 0 (live lanes only).
+// sL holds the element lengths between the first and last stage.
+template <typename T>
+__device__ __forceinline__ void element_plane_apply_stokes(int cp, int le, bool writer, bool live,
+                                                           const T *__restrict__ etap, long estride, long base,
+                                                           long X, const T *__restrict__ xc,
+                                                           const uint8_t *__restrict__ bcn,
+                                                           const T *__restrict__ u, const T *__restrict__ pe,
+                                                           T *__restrict__ ype, T (*sh)[kSlots][kEPW],
+                                                           T (*sL)[kEPW], T out[3][9]) {
+  PlaneRaw<T> r;
+  load_plane_nodes<T, true>(r, xc, u, bcn, base, X);
+  load_plane_eta<T>(r, etap, estride);
+  PCoupling<T> pcp;
+  // centroid and lengths: x14 - x12 and y16 - y10 lie in plane 1, z22 - z4
+  // joins the centre nodes of planes 2 and 0
+  if (writer) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      T sum = T(0);
+#pragma unroll
+      for (int n = 0; n < 9; n++) sum += r.x[d][n];
+      sh[cp][d][le] = sum;
+    }
+    sh[cp][3][le] = r.x[2][4];
+    sh[cp][4][le] = r.x[0][5] - r.x[0][3];
+    sh[cp][5][le] = r.x[1][7] - r.x[1][1];
+  }
+  __syncthreads();
+  const T Lx = sh[1][4][le], Ly = sh[1][5][le], Lz = sh[2][3][le] - sh[0][3][le];
+#pragma unroll
+  for (int d = 0; d < 3; d++) pcp.xb[d] = (sh[0][d][le] + sh[1][d][le] + sh[2][d][le]) * T(1.0 / 27.0);
+  if (writer && cp == 0) {
+    sL[0][le] = Lx;
+    sL[1][le] = Ly;
+    sL[2][le] = Lz;
+  }
+  pcp.pc[0] = pe[0];
+  pcp.pc[1] = pe[1] / Lx;
+  pcp.pc[2] = pe[2] / Ly;
+  pcp.pc[3] = pe[3] / Lz;
+#pragma unroll
+  for (int i = 0; i < 4; i++) pcp.acc[i] = T(0);
+  __syncthreads();
+  T Xq[3][9], Uq[3][9];
+  plane_forward<T, true>(cp, le, writer, r, sh, Xq, Uq);
+  plane_qp<T, true>(cp, le, writer, Xq, Uq, r.eta, sh, &pcp);
+  plane_backward<T>(cp, le, sh, out);
+  if (writer) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) sh[cp][i][le] = pcp.acc[i];
+  }
+  __syncthreads();
+  if (live && cp == 0) {
+    ype[0] = sh[0][0][le] + sh[1][0][le] + sh[2][0][le];
+#pragma unroll
+    for (int i = 1; i < 4; i++) ype[i] = (sh[0][i][le] + sh[1][i][le] + sh[2][i][le]) / sL[i - 1][le];
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -496,12 +586,14 @@ __device__ __forceinline__ void flush_plane(const TileArgs &a, long tile, const 
 
 // One tile's z-march (the work of one wave); shared by the single-box and
 // the batched MatMult kernels.
-template <typename T, bool kMask>
+template <typename T, bool kMask, bool kStokes = false>
 __device__ __forceinline__ void tile_march(const TileArgs &a, long tile, const T *__restrict__ xc,
                                            const T *__restrict__ eta, const uint8_t *__restrict__ bcn,
                                            const T *__restrict__ u, T *__restrict__ y, T *__restrict__ fbx,
                                            T *__restrict__ fby, T *__restrict__ fbz,
-                                           T (*sh)[kSlots][kEPW], T (*win)[3 * kWinNodes]) {
+                                           T (*sh)[kSlots][kEPW], T (*win)[3 * kWinNodes],
+                                           const T *__restrict__ p = nullptr, T *__restrict__ yp = nullptr,
+                                           T (*sL)[kEPW] = nullptr) {
   const TileBox tb = tile_box(a, tile);
   const int lane = threadIdx.x;
   const int c = lane / kEPW;
@@ -519,8 +611,14 @@ __device__ __forceinline__ void tile_march(const TileArgs &a, long tile, const T
     const long ek = tb.z0 + kz;
     const long base = 2L * ex + a.X * (2L * ey) + a.XY * (2 * ek + cp);
     T out[3][9];
-    element_plane_apply<T, kMask>(cp, le, c < 3, eta + tq_index(tile, a.ZC, kz, 9 * cp, le), kEPW, base,
-                                  a.X, xc, bcn, u, sh, out);
+    if constexpr (kStokes) {
+      const long e = ex + (long)a.mx * (ey + (long)a.my * ek);
+      element_plane_apply_stokes<T>(cp, le, c < 3, live, eta + tq_index(tile, a.ZC, kz, 9 * cp, le), kEPW, base,
+                                    a.X, xc, bcn, u, p + 4 * e, yp + 4 * e, sh, sL, out);
+    } else {
+      element_plane_apply<T, kMask>(cp, le, c < 3, eta + tq_index(tile, a.ZC, kz, 9 * cp, le), kEPW, base,
+                                    a.X, xc, bcn, u, sh, out);
+    }
     // Sum into the window.  For a fixed node slot (i,j) no two lanes touch
     // the same window entry, so the nine slots are added one after another
     // with plain read-add-write; the wave executes LDS operations in order
@@ -558,6 +656,21 @@ __global__ void __launch_bounds__(64)
   __shared__ T sh[3][kSlots][kEPW];
   __shared__ T win[3][3 * kWinNodes];
   tile_march<T, kMask>(a, xcd_tile(blockIdx.x, a.ntiles), xc, eta, bcn, u, y, fbx, fby, fbz, sh, win);
+}
+
+// Saddle-point MatMult: the A11 z-march with the pressure coupling; y holds
+// the velocity part, yp the pressure part (4 per element, written once).
+template <typename T>
+__global__ void __launch_bounds__(64)
+    stokes_tile_kernel(TileArgs a, const T *__restrict__ xc, const T *__restrict__ eta,
+                       const uint8_t *__restrict__ bcn, const T *__restrict__ u, const T *__restrict__ p,
+                       T *__restrict__ y, T *__restrict__ yp, T *__restrict__ fbx, T *__restrict__ fby,
+                       T *__restrict__ fbz) {
+  __shared__ T sh[3][kSlots][kEPW];
+  __shared__ T win[3][3 * kWinNodes];
+  __shared__ T sL[3][kEPW];
+  tile_march<T, true, true>(a, xcd_tile(blockIdx.x, a.ntiles), xc, eta, bcn, u, y, fbx, fby, fbz, sh, win, p, yp,
+                            sL);
 }
 
 // The same z-march, software-pipelined across layers: the raw node plane of
@@ -1148,6 +1261,24 @@ int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, bool faces) {
     PTMF_HIP(hipGetLastError());
     PTMF_TRY(launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
   }
+  return PTMF_OK;
+}
+
+int launch_stokes_mult(ptmf_op_s *op, const double *x, double *y) {
+  const Tiling &tl = op->tiling;
+  PTMF_CHECK(tl.TX * tl.TY <= kEPW && (2 * tl.TX + 1) * (2 * tl.TY + 1) <= kWinNodes && tl.ZC > 0,
+             PTMF_ERR_ARG, "stokes: invalid tiling %d x %d x %d", tl.TX, tl.TY, tl.ZC);
+  PTMF_CHECK(op->faces != nullptr, PTMF_ERR_NOT_SETUP, "stokes: face buffers not allocated");
+  PTMF_TRY(ensure_eta_tiled<double>(op));
+  const TileArgs a = tile_args(op);
+  const long ntiles = tl.ntiles, nu = 3 * op->box.nnodes;
+  double *fbx = (double *)op->faces, *fby = fbx + 3 * a.fbx_n * ntiles, *fbz = fby + 3 * a.fby_n * ntiles;
+  stokes_tile_kernel<double><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
+      a, op->d64.coords, op->et64, op->bcnode, x, x + nu, y, y + nu, fbx, fby, fbz);
+  PTMF_HIP(hipGetLastError());
+  a11_face_kernel<double><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(a, fbx, fby, fbz, y);
+  PTMF_HIP(hipGetLastError());
+  PTMF_TRY(launch_bc_copy<double>(op->bcidx, op->nbc, x, y, op->stream));
   return PTMF_OK;
 }
 

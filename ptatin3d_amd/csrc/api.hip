@@ -442,6 +442,14 @@ int ptmf_a21_mult(ptmf_op op, const double *u, double *yp) {
   return PTMF_OK;
 }
 
+int ptmf_stokes_mult(ptmf_op op, const double *x, double *y) {
+  PTMF_TRY(check_ready(op, true, true));
+  PTMF_CHECK(x && y, PTMF_ERR_ARG, "stokes_mult: null");
+  PTMF_TRY(ptmf::launch_stokes_mult(op, x, y));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  return PTMF_OK;
+}
+
 }  // extern "C"
 
 namespace {

@@ -32,6 +32,58 @@ __device__ __forceinline__ double pick3(int i, double v0, double v1, double v2) 
   return i == 0 ? v0 : (i == 1 ? v1 : v2);
 }
 
+// The 1-D tables as one lane sees them, in registers.  A lane that owns
+// quadrature point (qa, qb, qc) holds rows: B[d][r] = kB[q_d][r]; a lane that
+// owns node (i, j, k) holds columns: B[d][a] = kB[a][n_d].  tab_grad(T, x, y,
+// z) is then the reference gradient of the (node, point) pair whose other
+// index is (x, y, z), with static register indices when x, y, z are
+// compile-time (unrolled loops).
+struct LaneTab {
+  double B[3][3], D[3][3];
+};
+
+__device__ __forceinline__ void lane_tab_qp(int q, LaneTab &T) {
+  const int qd[3] = {q % 3, (q / 3) % 3, q / 9};
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      T.B[d][r] = pick3(qd[d], kB[0][r], kB[1][r], kB[2][r]);
+      T.D[d][r] = pick3(qd[d], kD[0][r], kD[1][r], kD[2][r]);
+    }
+}
+
+__device__ __forceinline__ void lane_tab_node(int n, LaneTab &T) {
+  const int nd[3] = {n % 3, (n / 3) % 3, n / 9};
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      T.B[d][a] = pick3(nd[d], kB[a][0], kB[a][1], kB[a][2]);
+      T.D[d][a] = pick3(nd[d], kD[a][0], kD[a][1], kD[a][2]);
+    }
+}
+
+__device__ __forceinline__ void tab_grad(const LaneTab &T, int x, int y, int z, double gr[3]) {
+  gr[0] = T.D[0][x] * T.B[1][y] * T.B[2][z];
+  gr[1] = T.B[0][x] * T.D[1][y] * T.B[2][z];
+  gr[2] = T.B[0][x] * T.B[1][y] * T.D[2][z];
+}
+
+// element of a parity class (color = px + 2 py + 4 pz) from its slot; no two
+// elements of a class share a node
+__device__ __forceinline__ long class_element(int mx, int my, int mz, int color, long slot, bool &live) {
+  const int px = color & 1, py = (color >> 1) & 1, pz = color >> 2;
+  const long ncx = (mx - px + 1) / 2, ncy = (my - py + 1) / 2, ncz = (mz - pz + 1) / 2;
+  live = slot < ncx * ncy * ncz;
+  if (!live) slot = ncx * ncy * ncz - 1;
+  return (2 * (slot % ncx) + px) + mx * ((2 * ((slot / ncx) % ncy) + py) + (long)my * (2 * (slot / (ncx * ncy)) + pz));
+}
+
+inline long class_count(const Box &b, int color) {
+  return (long)((b.mx - (color & 1) + 1) / 2) * ((b.my - ((color >> 1) & 1) + 1) / 2) * ((b.mz - (color >> 2) + 1) / 2);
+}
+
 __device__ void qp_geometry(const double (*xe)[3], int q, QpGeom &g) {
   const int qa = q % 3, qb = (q / 3) % 3, qc = q / 9;
   // the lane's rows of the 1-D tables, held in registers
@@ -119,14 +171,9 @@ __global__ void __launch_bounds__(64)
   __shared__ double sx[kEPB][27][3];
   __shared__ double sG[kEPB][27][10];
   const int h = threadIdx.x >> 5, t = threadIdx.x & 31;
-  // elements of one parity class (color = px + 2 py + 4 pz): no two of them
-  // share a node, so their sums go in with plain read-add-write
-  const int px = color & 1, py = (color >> 1) & 1, pz = color >> 2;
-  const long ncx = (mx - px + 1) / 2, ncy = (my - py + 1) / 2, ncz = (mz - pz + 1) / 2;
-  long slot = (long)blockIdx.x * kEPB + h;
-  const bool live = slot < ncx * ncy * ncz;
-  if (!live) slot = ncx * ncy * ncz - 1;
-  const long e = (2 * (slot % ncx) + px) + mx * ((2 * ((slot / ncx) % ncy) + py) + (long)my * (2 * (slot / (ncx * ncy)) + pz));
+  // elements of one parity class: their sums go in with plain read-add-write
+  bool live;
+  const long e = class_element(mx, my, mz, color, (long)blockIdx.x * kEPB + h, live);
   if (t < 27) {
     const long node = elem_node(mx, my, X, XY, e, t);
     for (int d = 0; d < 3; d++) sx[h][t][d] = xc[3 * node + d];
@@ -141,30 +188,21 @@ __global__ void __launch_bounds__(64)
   }
   __syncthreads();
   if (t < 27 && live) {
-    // columns (i, j, k) = node t of the 1-D tables, in registers
-    const int ni = t % 3, nj = (t / 3) % 3, nk = t / 9;
-    double Bi[3], Di[3], Bj[3], Dj[3], Bk[3], Dk[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      Bi[a] = pick3(ni, kB[a][0], kB[a][1], kB[a][2]);
-      Di[a] = pick3(ni, kD[a][0], kD[a][1], kD[a][2]);
-      Bj[a] = pick3(nj, kB[a][0], kB[a][1], kB[a][2]);
-      Dj[a] = pick3(nj, kD[a][0], kD[a][1], kD[a][2]);
-      Bk[a] = pick3(nk, kB[a][0], kB[a][1], kB[a][2]);
-      Dk[a] = pick3(nk, kD[a][0], kD[a][1], kD[a][2]);
-    }
+    LaneTab T;
+    lane_tab_node(t, T);
     double acc[3] = {0, 0, 0};
     // one quadrature plane per trip (the unrolled 9 points keep their
     // geometry loads in registers; all 27 at once would spill)
 #pragma unroll 1
     for (int qc = 0; qc < 3; qc++) {
-      const double bk = pick3(qc, Bk[0], Bk[1], Bk[2]), dk = pick3(qc, Dk[0], Dk[1], Dk[2]);
+      const double bk = pick3(qc, T.B[2][0], T.B[2][1], T.B[2][2]), dk = pick3(qc, T.D[2][0], T.D[2][1], T.D[2][2]);
 #pragma unroll
       for (int qb = 0; qb < 3; qb++)
 #pragma unroll
         for (int qa = 0; qa < 3; qa++) {
           const double *G = sG[h][qa + 3 * qb + 9 * qc];
-          const double gr[3] = {Di[qa] * Bj[qb] * bk, Bi[qa] * Dj[qb] * bk, Bi[qa] * Bj[qb] * dk};
+          const double gr[3] = {T.D[0][qa] * T.B[1][qb] * bk, T.B[0][qa] * T.D[1][qb] * bk,
+                                T.B[0][qa] * T.B[1][qb] * dk};
           double gp[3];
 #pragma unroll
           for (int k = 0; k < 3; k++) gp[k] = G[3 * k] * gr[0] + G[3 * k + 1] * gr[1] + G[3 * k + 2] * gr[2];
@@ -258,14 +296,13 @@ __global__ void __launch_bounds__(128)
 // (MFStokesWrapper_A12, stokes_operators_mf.c:788-856 with
 // MatMultMF_Stokes_MixedFEM3d_A12, stokes_q2p1_mf_operators_def.c:423-516)
 __global__ void __launch_bounds__(64)
-    a12_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
+    a12_kernel(int mx, int my, int mz, long nel, long X, long XY, int color, const double *__restrict__ xc,
                const double *__restrict__ p, double *__restrict__ yu) {
   __shared__ double sx[kEPB][27][3];
   __shared__ double sG[kEPB][27][10];
   const int h = threadIdx.x >> 5, t = threadIdx.x & 31;
-  long e = (long)blockIdx.x * kEPB + h;
-  const bool live = e < nel;
-  if (!live) e = nel - 1;
+  bool live;  // one parity class per launch: plain read-add-write below
+  const long e = class_element(mx, my, mz, color, (long)blockIdx.x * kEPB + h, live);
   if (t < 27) {
     const long node = elem_node(mx, my, X, XY, e, t);
     for (int d = 0; d < 3; d++) sx[h][t][d] = xc[3 * node + d];
@@ -284,16 +321,26 @@ __global__ void __launch_bounds__(64)
   }
   __syncthreads();
   if (t < 27 && live) {
+    LaneTab T;
+    lane_tab_node(t, T);
     double acc[3] = {0, 0, 0};
-    for (int q = 0; q < 27; q++) {
-      double gr[3];
-      grad_ref(t, q, gr);
-      const double *G = sG[h][q];
-      for (int k = 0; k < 3; k++)
-        acc[k] += G[9] * (G[3 * k] * gr[0] + G[3 * k + 1] * gr[1] + G[3 * k + 2] * gr[2]);
+#pragma unroll 1
+    for (int qc = 0; qc < 3; qc++) {
+      const double bk = pick3(qc, T.B[2][0], T.B[2][1], T.B[2][2]), dk = pick3(qc, T.D[2][0], T.D[2][1], T.D[2][2]);
+#pragma unroll
+      for (int qb = 0; qb < 3; qb++)
+#pragma unroll
+        for (int qa = 0; qa < 3; qa++) {
+          const double *G = sG[h][qa + 3 * qb + 9 * qc];
+          const double gr[3] = {T.D[0][qa] * T.B[1][qb] * bk, T.B[0][qa] * T.D[1][qb] * bk,
+                                T.B[0][qa] * T.B[1][qb] * dk};
+#pragma unroll
+          for (int k = 0; k < 3; k++)
+            acc[k] += G[9] * (G[3 * k] * gr[0] + G[3 * k + 1] * gr[1] + G[3 * k + 2] * gr[2]);
+        }
     }
     const long node = elem_node(mx, my, X, XY, e, t);
-    for (int c = 0; c < 3; c++) atomicAdd(&yu[3 * node + c], acc[c]);
+    for (int c = 0; c < 3; c++) yu[3 * node + c] += acc[c];
   }
 }
 
@@ -327,9 +374,13 @@ __global__ void __launch_bounds__(64)
     double Np[4];
     p1_basis(sx[h], g.xq, Np);
     double div = 0.0;
+    LaneTab T;
+    lane_tab_qp(t, T);
+#pragma unroll
     for (int n = 0; n < 27; n++) {
       double gr[3];
-      grad_ref(n, t, gr);
+      tab_grad(T, n % 3, (n / 3) % 3, n / 9, gr);
+#pragma unroll
       for (int k = 0; k < 3; k++)
         div += su[h][n][k] * (g.G[k][0] * gr[0] + g.G[k][1] * gr[1] + g.G[k][2] * gr[2]);
     }
@@ -454,9 +505,12 @@ __global__ void __launch_bounds__(64)
     QpGeom g;
     qp_geometry(sx[h], t, g);
     double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // L[l][k] = d u_l / d x_k
+    LaneTab T;
+    lane_tab_qp(t, T);
+#pragma unroll
     for (int n = 0; n < 27; n++) {
       double gr[3], gp[3];
-      grad_ref(n, t, gr);
+      tab_grad(T, n % 3, (n / 3) % 3, n / 9, gr);
       for (int k = 0; k < 3; k++) gp[k] = g.G[k][0] * gr[0] + g.G[k][1] * gr[1] + g.G[k][2] * gr[2];
       for (int l = 0; l < 3; l++)
         for (int k = 0; k < 3; k++) L[l][k] += su[h][n][l] * gp[k];
@@ -491,8 +545,7 @@ int launch_a11_diag(const ptmf_op_s *op, double *d) {
   PTMF_TRY(init_constants());
   const Box &b = op->box;
   for (int color = 0; color < 8; color++) {
-    const long ncol = (long)((b.mx - (color & 1) + 1) / 2) * ((b.my - ((color >> 1) & 1) + 1) / 2) *
-                      ((b.mz - (color >> 2) + 1) / 2);
+    const long ncol = class_count(b, color);
     if (ncol == 0) continue;
     const long nblk = (ncol + kEPB - 1) / kEPB;
     a11_diag_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
@@ -546,10 +599,14 @@ int launch_powerlaw(const ptmf_op_s *op, const double *u, const double *B, doubl
 int launch_a12(const ptmf_op_s *op, const double *p, double *yu) {
   PTMF_TRY(init_constants());
   const Box &b = op->box;
-  const long nblk = (b.nel + kEPB - 1) / kEPB;
-  a12_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(b.mx, b.my, b.nel, b.X,
-                                                                 b.X * b.Y, op->d64.coords, p, yu);
-  PTMF_HIP(hipGetLastError());
+  for (int color = 0; color < 8; color++) {
+    const long ncol = class_count(b, color);
+    if (ncol == 0) continue;
+    const long nblk = (ncol + kEPB - 1) / kEPB;
+    a12_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(b.mx, b.my, b.mz, b.nel, b.X, b.X * b.Y,
+                                                                   color, op->d64.coords, p, yu);
+    PTMF_HIP(hipGetLastError());
+  }
   return PTMF_OK;
 }
 

@@ -1271,14 +1271,9 @@ namespace {
 // Saddle point K = [A11 A12; A21 0] on [u | p] (3N + 4E values):
 // yu = A11 xu + A12 xp with Dirichlet rows y = x, yp = A21 xu
 // (MatMult of the reference's MatNest Stokes operator, stokes_operators.c).
+// y = K x in one pass over the elements (the fused saddle-point kernel)
 int stokes_apply(ptmf_mg_s *mg, const double *x, double *y) {
-  MGLevel &F = mg->lev[mg->L - 1];
-  const long nu = F.n;
-  PTMF_TRY(launch_a11_mult<double>(F.op, x, y, true));
-  PTMF_TRY(launch_a12(F.op, x + nu, y));
-  PTMF_TRY(launch_bc_copy<double>(F.op->bcidx, F.op->nbc, x, y, mg->stream));
-  PTMF_TRY(launch_a21(F.op, x, y + nu));
-  return PTMF_OK;
+  return launch_stokes_mult(mg->lev[mg->L - 1].op, x, y);
 }
 
 // Upper block-triangular preconditioner [A11 A12; 0 S]^-1 with S the scaled

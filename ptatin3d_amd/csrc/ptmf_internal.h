@@ -145,6 +145,7 @@ template <typename T>
 int launch_bc_set(const long *idx, long n, T v, T *y, hipStream_t s);
 int launch_copy16(const void *x, void *y, long n16, hipStream_t s);
 int launch_eta_qpmajor(long nel, const double *em, double *q64, float *q32, hipStream_t s);
+int launch_stokes_mult(ptmf_op_s *op, const double *x, double *y);
 int launch_powerlaw(const ptmf_op_s *op, const double *u, const double *B, double nexp, double eta_min,
                     double eta_max, double *eta);
 int launch_a11_diag(const ptmf_op_s *op, double *d);

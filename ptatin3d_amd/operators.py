@@ -179,6 +179,20 @@ class MatStokesMF(MatA11MF):
         """MatMult_MFStokes_A21: yp = B^T mask(u)."""
         call("ptmf_a21_mult", self.handle, u.ptr, yp.ptr)
 
+    def mult_stokes(self, x: DeviceArray, y: DeviceArray) -> None:
+        """MatMult_MFStokes_A: y = [A11 A12; A21 0] [u | p] in one pass."""
+        call("ptmf_stokes_mult", self.handle, x.ptr, y.ptr)
+
+    def stokes_host(self, u: np.ndarray, p: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        x = DeviceArray.from_host(np.concatenate([np.asarray(u, np.float64), np.asarray(p, np.float64)]))
+        y = DeviceArray(x.n)
+        self.mult_stokes(x, y)
+        out = y.download()
+        x.free()
+        y.free()
+        nu = 3 * self.nnodes
+        return out[:nu], out[nu:]
+
     def a12_host(self, p: np.ndarray) -> np.ndarray:
         dp = DeviceArray.from_host(np.asarray(p, dtype=np.float64))
         dy = DeviceArray(3 * self.nnodes)
