@@ -73,6 +73,9 @@ def main() -> int:
     ap.add_argument("--solve-m", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--batch", type=int, default=256, help="boxes in the batched-MatMult leg (0: skip)")
+    ap.add_argument("--stokes", type=int, nargs=3, metavar=("MX", "MY", "MZ"), default=None,
+                    help="run the rift saddle-point leg on this box (e.g. 192 192 96)")
+    ap.add_argument("--stokes-maxit", type=int, default=600)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -210,6 +213,8 @@ def main() -> int:
         out["batch"] = batch_leg(args.batch)
     if not args.no_solve:
         out["solve"] = solve_leg(args.solve_m)
+    if args.stokes:
+        out["stokes"] = stokes_leg(tuple(args.stokes), args.stokes_maxit)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -303,6 +308,75 @@ def solve_leg(m: int) -> dict:
     for tag in ("fp64", "fp32"):
         res[tag]["rel_err_vs_tight"] = float(np.linalg.norm(runs[tag] - ref) / np.linalg.norm(ref))
     res["speedup_fp32_vs_fp64"] = round(res["fp64"]["seconds"] / res["fp32"]["seconds"], 3)
+    return res
+
+
+def stokes_leg(dims, maxit: int) -> dict:
+    """Rift saddle point: a 1e3 crust (z > 0.75) over a 1e0 mantle with a
+    seeded 1e-3 notch, power-law n = 3 viscosity of a seeded extension
+    velocity at the quadrature points (clipped to [1e-3, 1e3]), gravity load;
+    free-slip sides and base, extension velocity on the x-faces, free top.
+    FGMRES(40) in FP64 on [u | p] with the fused saddle-point MatMult and the
+    upper block-triangular preconditioner (scaled viscous-mass Schur
+    approximation, one 5-level V-cycle for A11: Chebyshev(4)-Jacobi, FP32 or
+    FP64).  Reports the fused MatMult rate, then both solves to 1e-8 or maxit."""
+    from ptatin3d_amd import fields as F
+    from ptatin3d_amd._lib import DeviceArray, call
+    from ptatin3d_amd.multigrid import (default_options, powerlaw_viscosity, rift_hierarchy,
+                                        stokes_rhs_lifted, stokes_solve_timed)
+    from ptatin3d_amd.operators import MatStokesMF
+
+    mx, my, mz = dims
+    nl = 1
+    while all(((d >> nl) << nl) == d and (d >> nl) >= 2 for d in dims) and nl < 5:
+        nl += 1
+    c = F.config5(mx, my, mz)
+    eta = powerlaw_viscosity(dims, c["coords"], c["u0"], c["B"], c["nexp"], c["eta_min"], c["eta_max"])
+    bu, bp = stokes_rhs_lifted(dims, c["coords"], eta, c["bc"], c["g"], c["f"])
+    E, nu = mx * my * mz, bu.size
+    res = {"workload": f"rift saddle point {mx}x{my}x{mz} Q2-P1disc: 1e3 crust / 1e0 mantle / 1e-3 notch, "
+                       "power-law n=3 viscosity of a seeded extension velocity (contrast 1e6), gravity, "
+                       "free-slip sides and base, x-face extension, free top",
+           "velocity_dofs": nu, "pressure_dofs": 4 * E, "levels": nl,
+           "krylov": "FGMRES(40), FP64, fused saddle-point MatMult",
+           "preconditioner": "upper block-triangular: scaled viscous-mass Schur, one V-cycle for A11"}
+    # fused saddle-point MatMult throughput
+    A = MatStokesMF(mx, my, mz).setup(c["coords"], eta, c["bc"])
+    x = DeviceArray.from_host(np.concatenate([bu, bp]))
+    y = DeviceArray(x.n)
+    for _ in range(3):
+        A.mult_stokes(x, y)
+    call("ptmf_device_synchronize")
+    t0 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        A.mult_stokes(x, y)
+    call("ptmf_device_synchronize")
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    res["matmult_ms"] = round(ms, 3)
+    res["matmult_GDoF_s"] = round((nu + 4 * E) / (ms * 1e-3) / 1e9, 3)
+    x.free()
+    y.free()
+    A.destroy()
+    bn = float(np.linalg.norm(np.concatenate([bu, bp])))
+    sol = {}
+    for tag, prec in (("fp32", 32), ("fp64", 64)):
+        t0 = time.perf_counter()
+        mg = rift_hierarchy(c, nl, eta, default_options(smoother_precision=prec, pre_its=4, post_its=4,
+                                                          esteig=(0.0, 0.1, 0.0, 1.1)))
+        ts = time.perf_counter() - t0
+        u, p, its, rn, hist, secs = stokes_solve_timed(mg, bu, bp, rtol=1e-8, maxit=maxit, restart=40)
+        mg.destroy()
+        sol[tag] = (u, p)
+        res[tag] = {"its": its, "seconds": round(secs, 3), "ms_per_it": round(1e3 * secs / max(its, 1), 2),
+                    "setup_seconds": round(ts, 3), "rel_residual": float(rn / bn),
+                    "converged": bool(rn <= 1e-8 * bn)}
+        print(f"# stokes leg {tag}: {its} its, {secs:.2f} s, rel residual {rn / bn:.2e}",
+              file=sys.stderr, flush=True)
+    res["rel_diff_fp32_vs_fp64"] = {
+        "u": float(np.linalg.norm(sol["fp32"][0] - sol["fp64"][0]) / np.linalg.norm(sol["fp64"][0])),
+        "p": float(np.linalg.norm(sol["fp32"][1] - sol["fp64"][1]) / np.linalg.norm(sol["fp64"][1]))}
+    res["speedup_fp32_vs_fp64_per_it"] = round(res["fp64"]["ms_per_it"] / res["fp32"]["ms_per_it"], 3)
     return res
 
 

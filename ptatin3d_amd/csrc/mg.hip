@@ -1121,8 +1121,8 @@ int assemble_galerkin(ptmf_mg_s *mg, int l) {
 int assemble_dense(ptmf_mg_s *mg) {
   MGLevel &L = mg->lev[0];
   const long n = L.n;
-  PTMF_CHECK(n <= 24000, PTMF_ERR_UNSUPPORTED,
-             "coarsest level has %ld dofs; the dense coarse solve takes at most 24000", n);
+  PTMF_CHECK(n <= 32768, PTMF_ERR_UNSUPPORTED,
+             "coarsest level has %ld dofs; the dense coarse solve takes at most 32768", n);
   mg->ndense = (int)n;
   if (!mg->Adense) PTMF_TRY(dalloc((void **)&mg->Adense, sizeof(double) * n * n));
   hipStream_t s = mg->stream;

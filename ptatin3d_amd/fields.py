@@ -217,3 +217,82 @@ def config3(m=128, seed=SEED):
     bc = dirichlet_all_faces(m, m, m)
     b = smooth_body_force(coords, m, m, m, rng)
     return dict(mx=m, my=m, mz=m, coords=coords, eta=eta, bc=bc, b=b)
+
+
+# ---- saddle-point rift workload ---------------------------------------------
+
+RIFT_LEN = (2.0, 2.0, 1.0)  # domain [0,2] x [0,2] x [0,1], z up
+
+
+def rift_bc(mx, my, mz):
+    """Free-slip sides and base, extension on the x-faces, free top surface:
+    u_x fixed on x = 0 and x = 2 (values -1 and +1), u_y = 0 on the y-faces,
+    u_z = 0 on the base.  Returns the 3N Dirichlet mask and the 3N vector of
+    boundary values (zero off the boundary)."""
+    X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+    m = np.zeros((Z, Y, X, 3), dtype=np.uint8)
+    g = np.zeros((Z, Y, X, 3))
+    m[:, :, 0, 0] = m[:, :, -1, 0] = 1
+    g[:, :, 0, 0], g[:, :, -1, 0] = -1.0, 1.0
+    m[:, 0, :, 1] = m[:, -1, :, 1] = 1
+    m[0, :, :, 2] = 1
+    return m.reshape(-1), g.reshape(-1)
+
+
+def rift_prefactor(mx, my, mz, nexp=3.0, rng=None) -> np.ndarray:
+    """Power-law prefactor B at the quadrature points (element-major 27E):
+    a crust (z > 0.75, eta 1e3) over a mantle (eta 1), and a seeded weak notch
+    (eta 1e-3) at the base of the crust across the whole y extent; B = eta *
+    e_ref^(1 - 1/n) with the reference strain rate e_ref = 1 of the imposed
+    extension, so eta(u) equals the layer value at e_ref."""
+    rng = rng or np.random.default_rng(SEED + 7)
+    gx, gy, gz = box_qp_axes(mx, my, mz)
+    gx, gy, gz = gx * RIFT_LEN[0], gy * RIFT_LEN[1], gz * RIFT_LEN[2]
+    xc = rng.uniform(0.9, 1.1)
+    hw = rng.uniform(0.04, 0.08)
+    zl = np.where(gz > 0.75, 1e3, 1.0)  # (mz, 3)
+    notch_x = np.abs(gx - xc) <= hw  # (mx, 3)
+    notch_z = (gz > 0.75) & (gz < 0.85)  # (mz, 3)
+    eta = np.empty((mz, my, mx, 3, 3, 3))
+    eta[...] = zl[:, None, None, :, None, None]
+    inside = notch_z[:, None, None, :, None, None] & notch_x[None, None, :, None, None, :]
+    eta[np.broadcast_to(inside, eta.shape)] = 1e-3
+    return eta.reshape(-1)  # e_ref = 1: B = eta
+
+
+def rift_initial_velocity(coords, rng=None) -> np.ndarray:
+    """Seeded initial velocity for the strain-rate-dependent viscosity: the
+    divergence-free extension u = (x - 1, 0, -z) plus a seeded smooth
+    perturbation of amplitude 0.2."""
+    rng = rng or np.random.default_rng(SEED + 8)
+    c = coords.reshape(-1, 3)
+    x, y, z = c[:, 0], c[:, 1], c[:, 2]
+    k = rng.integers(1, 4, size=(3, 3))
+    ph = rng.uniform(0, 2 * np.pi, size=(3, 3))
+    u = np.stack([x - 1.0, np.zeros_like(x), -z], axis=1)
+    for d in range(3):
+        u[:, d] += 0.2 * (np.sin(np.pi * k[d, 0] * x / 2 + ph[d, 0]) * np.sin(np.pi * k[d, 1] * y / 2 + ph[d, 1])
+                          * np.sin(np.pi * k[d, 2] * z + ph[d, 2]))
+    return u.reshape(-1)
+
+
+def rift_body_force(coords, mx, my, mz) -> np.ndarray:
+    """Gravity load -rho g e_z (rho 2.7 in the crust z > 0.75, 3.3 below,
+    g = 1) lumped to the nodes with the nodal volume of the box."""
+    c = coords.reshape(-1, 3)
+    f = np.zeros_like(c)
+    vol = RIFT_LEN[0] * RIFT_LEN[1] * RIFT_LEN[2] / (8.0 * mx * my * mz)
+    f[:, 2] = -np.where(c[:, 2] > 0.75, 2.7, 3.3) * vol
+    return f.reshape(-1)
+
+
+def config5(mx=192, my=192, mz=96, seed=SEED):
+    """Rift saddle point: 192 x 192 x 96 elements on [0,2]x[0,2]x[0,1]."""
+    rng = np.random.default_rng(seed)
+    coords = box_coords(mx, my, mz, hi=RIFT_LEN)
+    B = rift_prefactor(mx, my, mz, rng=rng)
+    u0 = rift_initial_velocity(coords, rng)
+    bc, g = rift_bc(mx, my, mz)
+    f = rift_body_force(coords, mx, my, mz)
+    return dict(mx=mx, my=my, mz=mz, coords=coords, B=B, u0=u0, bc=bc, g=g, f=f, nexp=3.0,
+                eta_min=1e-3, eta_max=1e3)

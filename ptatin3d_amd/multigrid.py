@@ -180,7 +180,8 @@ def solve_timed(mg: GMG, b: np.ndarray, rtol=1e-8, maxit=200, restart=30, krylov
     return x, its, rn, hist, t1 - t0
 
 
-def stokes_solve_timed(mg: GMG, bu: np.ndarray, bp: np.ndarray, rtol=1e-8, maxit=500, restart=50):
+def stokes_solve_timed(mg: GMG, bu: np.ndarray, bp: np.ndarray, rtol=1e-8, maxit=500, restart=50,
+                       history_len=1024):
     """Saddle-point solve from x = 0; returns (u, p, its, rnorm, history, seconds)."""
     nu = bu.size
     b = np.concatenate([bu, bp])
@@ -189,10 +190,57 @@ def stokes_solve_timed(mg: GMG, bu: np.ndarray, bp: np.ndarray, rtol=1e-8, maxit
     dx.zero()
     call("ptmf_device_synchronize")
     t0 = time.perf_counter()
-    its, rn, hist = mg.stokes_fgmres(db, dx, rtol, maxit, restart)
+    its, rn, hist = mg.stokes_fgmres(db, dx, rtol, maxit, restart, history_len)
     call("ptmf_device_synchronize")
     t1 = time.perf_counter()
     x = dx.download()
     db.free()
     dx.free()
     return x[:nu], x[nu:], its, rn, hist, t1 - t0
+
+
+# -- workloads with non-homogeneous boundary values ----------------------------
+
+def powerlaw_viscosity(dims, coords, u, B, nexp, eta_min, eta_max) -> np.ndarray:
+    """Quadrature-point power-law viscosity of velocity u, evaluated on the
+    device (element-major 27E, host copy)."""
+    from .operators import MatA11MF
+    mx, my, mz = dims
+    E = mx * my * mz
+    A = MatA11MF(mx, my, mz).setup(coords, np.ones(27 * E), np.zeros(3 * F.nnodes(mx, my, mz), np.uint8))
+    du, dB, de = DeviceArray.from_host(u), DeviceArray.from_host(B), DeviceArray(27 * E)
+    A.powerlaw_viscosity(du, dB, nexp, eta_min, eta_max, de)
+    eta = de.download()
+    for d in (du, dB, de):
+        d.free()
+    A.destroy()
+    return eta
+
+
+def stokes_rhs_lifted(dims, coords, eta, bc, g, f):
+    """Right-hand side of K [u|p] = b for Dirichlet values g on the masked
+    dofs: b_u = f - A11 g, b_p = -A21 g with the unmasked operators, then
+    b_u = g on the Dirichlet dofs (the rows of K there are the identity)."""
+    from .operators import MatStokesMF
+    mx, my, mz = dims
+    A = MatStokesMF(mx, my, mz).setup(coords, eta, np.zeros_like(bc))
+    yu, yp = A.stokes_host(g, np.zeros(4 * mx * my * mz))
+    A.destroy()
+    m = np.asarray(bc).astype(bool)
+    bu = f - yu
+    bu[m] = g[m]
+    return bu, -yp
+
+
+def rift_hierarchy(c, nlevels, eta, opt: MGOptions, coarsen="qp-arithmetic", inner_its=1, inner_rtol=0.0):
+    """GMG hierarchy of the rift saddle point (free-slip sides and base,
+    x-face extension on every level) with the saddle-point preconditioner."""
+    mg = GMG(c["mx"], c["my"], c["mz"], nlevels)
+    mg.set_coordinates(c["coords"])
+    mg.set_viscosity(nlevels - 1, eta)
+    mg.coarsen_viscosity(coarsen)
+    for l, d in enumerate(mg.dims):
+        mg.set_dirichlet(l, F.rift_bc(*d)[0])
+    mg.setup(opt)
+    mg.stokes_setup(inner_its, inner_rtol)
+    return mg

@@ -103,3 +103,37 @@ def test_fused_stokes_mult_matches_oracle(gpu, m, tiling):
     assert rel_inf(yu, ou) <= 1e-12
     assert rel_inf(yp, op) <= 1e-12
     A.destroy()
+
+
+def test_rift_saddle_point_with_boundary_values(gpu):
+    """Rift workload (free-slip sides and base, extension on the x-faces, free
+    top, power-law viscosity of a seeded velocity, contrast 1e6) at 16x16x8:
+    the lifted right-hand side gives a solution that carries the boundary
+    values and satisfies the unconstrained equations at the free dofs, all
+    recomputed with the oracle's operators."""
+    from ptatin3d_amd.multigrid import powerlaw_viscosity, rift_hierarchy, stokes_rhs_lifted
+
+    dims = (16, 16, 8)
+    c = F.config5(*dims)
+    eta = powerlaw_viscosity(dims, c["coords"], c["u0"], c["B"], c["nexp"], c["eta_min"], c["eta_max"])
+    assert rel_inf(eta, O.powerlaw_eta(*dims, c["coords"], c["u0"], c["B"], 3.0, 1e-3, 1e3)) <= 1e-12
+    assert eta.min() == 1e-3 and eta.max() == 1e3
+    bu, bp = stokes_rhs_lifted(dims, c["coords"], eta, c["bc"], c["g"], c["f"])
+    mg = rift_hierarchy(c, 3, eta, default_options(smoother_precision=32, pre_its=4, post_its=4,
+                                                    esteig=(0.0, 0.1, 0.0, 1.1)))
+    u, p, its, rn, hist, secs = stokes_solve_timed(mg, bu, bp, rtol=1e-8, maxit=300, restart=100)
+    mg.destroy()
+    bn = np.linalg.norm(np.concatenate([bu, bp]))
+    assert rn <= 1e-8 * bn, its
+    m = c["bc"].astype(bool)
+    # the identity rows carry the boundary values up to the solver tolerance
+    assert np.linalg.norm(u[m] - c["g"][m]) <= 1e-8 * bn
+    # masked operator residual
+    yu, yp = _oracle_stokes(dims, c["coords"], eta, c["bc"], u, p)
+    assert np.linalg.norm(np.concatenate([bu - yu, bp - yp])) <= 2e-8 * bn
+    # unconstrained equations: A11 u + A12 p = f at the free dofs, A21 u = 0
+    z = np.zeros_like(c["bc"])
+    ru = O.a11_tensor(*dims, c["coords"], eta, u) + O.a12_matmult(*dims, c["coords"], z, p) - c["f"]
+    rp = O.a21_matmult(*dims, c["coords"], z, u)
+    assert np.linalg.norm(ru[~m]) <= 2e-8 * bn
+    assert np.linalg.norm(rp) <= 2e-8 * bn
