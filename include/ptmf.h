@@ -126,6 +126,16 @@ int ptmf_op_set_viscosity_device(ptmf_op op, const double *eta);
  * result to ptmf_op_set_viscosity_device for a Picard update. */
 int ptmf_op_powerlaw_viscosity(ptmf_op op, const double *u, const double *B, double nexp, double eta_min,
                                double eta_max, double *eta);
+/* Material-point viscosity at the quadrature points of op's box: point p
+ * lies in element elem[p] (64-bit) at reference coordinates xi[3p..3p+2] with
+ * value val[p].  Nodal field = sum_p w_p val_p / sum_p w_p with the Q1 corner
+ * basis at the 8 corners and weight 1 at the other 19 nodes, then Q1
+ * interpolation to the quadrature points clamped to the element's nodal range
+ * (SwarmUpdateGaussPropertiesLocalL2Projection_Q1_MPntPStokes,
+ * material_point_utils.c:604-705,842-890).  Device arrays; eta is 27E
+ * element-major, ready for ptmf_op_set_viscosity_device. */
+int ptmf_op_project_points(ptmf_op op, long npts, const long *elem, const double *xi, const double *val,
+                           double *eta);
 /* Work decomposition of the MatMult kernel: TX x TY element tiles (TX*TY <= 21)
  * marching through ZC element layers.  A default is chosen at creation; the
  * autotuner overrides it.  No reference counterpart (the CUDA backend's

@@ -174,3 +174,12 @@ def q1_projection_hierarchy(dims, eta_fine, inject=False):
         nodal = q1_restrict_scaled(*dims[l - 1], nodal, inject)
         out[l - 1] = q1_to_qp(*dims[l - 1], nodal)
     return out
+
+
+def q1_nodal_from_points(mx, my, mz, elem, xi, val):
+    """Fine nodal field from material points (element, reference coordinates, value)."""
+    out = np.zeros(nn(mx, my, mz))
+    el = np.ascontiguousarray(elem, dtype=np.int64)
+    load().orc_q1_nodal_from_points(mx, my, mz, C.c_long(el.size), el.ctypes.data_as(C.c_void_p),
+                                    _p(_f64(xi)), _p(_f64(val)), _p(out))
+    return out

@@ -880,3 +880,28 @@ void orc_q1_to_qp(int mx, int my, int mz, const double *nodal, double *eta)
     }
   }
 }
+
+/* Marker version of the fine nodal field: point p of element elem[p] at
+ * reference coordinates xi[3p..3p+2] carries val[p]; same weights as the
+ * quadrature-point version, accumulated point by point
+ * (_SwarmUpdateGaussPropertiesLocalL2ProjectionQ1_MPntPStokes,
+ * material_point_utils.c:604-705). */
+void orc_q1_nodal_from_points(int mx, int my, int mz, long npts, const long *elem, const double *xi,
+                              const double *val, double *nodal)
+{
+  box_t b = mkbox(mx, my, mz);
+  double *B = (double *)calloc((size_t)b.nnodes, sizeof(double));
+  memset(nodal, 0, sizeof(double) * b.nnodes);
+  for (long p = 0; p < npts; p++) {
+    double w[27];
+    long nid[27];
+    q1proj_weights(&xi[3 * p], w);
+    elnodes(&b, elem[p], nid);
+    for (int n = 0; n < 27; n++) {
+      nodal[nid[n]] += w[n] * val[p];
+      B[nid[n]] += w[n];
+    }
+  }
+  for (long n = 0; n < b.nnodes; n++) nodal[n] /= B[n];
+  free(B);
+}

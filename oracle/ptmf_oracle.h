@@ -111,6 +111,12 @@ void orc_powerlaw_eta(int mx, int my, int mz, const double *coords, const double
 void orc_q1_nodal_from_qp(int mx, int my, int mz, const double *eta, double *nodal);
 void orc_q1_restrict_scaled(int mxc, int myc, int mzc, int inject, const double *nf, double *nc);
 void orc_q1_to_qp(int mx, int my, int mz, const double *nodal, double *eta);
+/* The fine nodal field from material points instead of quadrature points:
+ * point p in element elem[p] at reference coordinates xi[3p..3p+2] with value
+ * val[p] (_SwarmUpdateGaussPropertiesLocalL2ProjectionQ1_MPntPStokes,
+ * material_point_utils.c:604-705). */
+void orc_q1_nodal_from_points(int mx, int my, int mz, long npts, const long *elem, const double *xi,
+                              const double *val, double *nodal);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_op_set_coordinates_device": (_I, [_P, _P]),
     "ptmf_op_set_viscosity_device": (_I, [_P, _P]),
     "ptmf_op_powerlaw_viscosity": (_I, [_P, _P, _P, _D, _D, _D, _P]),
+    "ptmf_op_project_points": (_I, [_P, _L, _P, _P, _P, _P]),
     "ptmf_op_state": (_I, [_P, C.POINTER(C.c_ulong)]),
     "ptmf_op_set_tiling": (_I, [_P, _I, _I, _I]),
     "ptmf_op_set_kernel_variant": (_I, [_P, _I]),

@@ -555,6 +555,38 @@ __global__ void k_q1_to_qp(int mx, int my, int mz, const double *__restrict__ no
   }
 }
 
+// the same nodal field from material points: point p of element elem[p] at
+// reference coordinates xi[3p..] with value val[p] adds its weights to the
+// 27 nodes of its element (_SwarmUpdateGaussPropertiesLocalL2Projection
+// Q1_MPntPStokes, material_point_utils.c:604-705); FP64 atomics, so the sums
+// are formed in no fixed order
+__global__ void k_q1_points_accum(long npts, int mx, int my, const long *__restrict__ elem,
+                                  const double *__restrict__ xi, const double *__restrict__ val,
+                                  double *__restrict__ A, double *__restrict__ B) {
+  const long X = 2L * mx + 1, XY = X * (2L * my + 1);
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < npts; p += (long)gridDim.x * blockDim.x) {
+    const long e = elem[p];
+    const long ei = e % mx, ej = (e / mx) % my, ek = e / ((long)mx * my);
+    const long base = 2 * ei + X * (2 * ej) + XY * (2 * ek);
+    const double x0 = xi[3 * p], x1 = xi[3 * p + 1], x2 = xi[3 * p + 2], v = val[p];
+    const double f0[2] = {0.5 * (1.0 - x0), 0.5 * (1.0 + x0)}, f1[2] = {0.5 * (1.0 - x1), 0.5 * (1.0 + x1)},
+                 f2[2] = {0.5 * (1.0 - x2), 0.5 * (1.0 + x2)};
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+          const double w = ((i | j | k) & 1) ? 1.0 : f0[i >> 1] * f1[j >> 1] * f2[k >> 1];
+          const long node = base + i + X * j + XY * k;
+          atomicAdd(&A[node], w * v);
+          atomicAdd(&B[node], w);
+        }
+  }
+}
+
+__global__ void k_divide(long n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    c[t] = a[t] / b[t];
+}
+
 // ---------------------------------------------------------------------------
 // Galerkin coarse operators stored as 125-point nodal stencils with 3x3
 // blocks, offset-major for coalesced reads: S[((o * 9) + 3 r + c) * N + m] is
@@ -1837,6 +1869,37 @@ int ptmf_mg_stokes_apply(ptmf_mg mg, const double *x, double *y) {
   PTMF_TRY(ptmf::stokes_apply(mg, x, y));
   PTMF_HIP(hipStreamSynchronize(mg->stream));
   return PTMF_OK;
+}
+
+
+// Material-point viscosity projected to the quadrature points of op's box:
+// Q1-weighted nodal field from the points, then clamped corner interpolation
+// (element-major eta, 27E).  Device arrays; elem holds 64-bit element indices.
+int ptmf_op_project_points(ptmf_op op, long npts, const long *elem, const double *xi, const double *val,
+                           double *eta) {
+  PTMF_CHECK(op && eta && (npts == 0 || (elem && xi && val)), PTMF_ERR_ARG, "project_points: null");
+  PTMF_CHECK(npts >= 0, PTMF_ERR_ARG, "project_points: %ld points", npts);
+  const ptmf::Box &b = op->box;
+  double *A = nullptr, *B = nullptr;
+  PTMF_TRY(ptmf::dalloc((void **)&A, sizeof(double) * 2 * b.nnodes));
+  B = A + b.nnodes;
+  hipStream_t s = op->stream;
+  int st = PTMF_OK;
+  if (hipMemsetAsync(A, 0, sizeof(double) * 2 * b.nnodes, s) != hipSuccess)
+    st = ptmf_set_error(PTMF_ERR_DEVICE, "project_points: memset");
+  if (st == PTMF_OK && npts > 0) {
+    ptmf::k_q1_points_accum<<<ptmf::nblocks(npts), ptmf::kBS, 0, s>>>(npts, b.mx, b.my, elem, xi, val, A, B);
+    if (hipGetLastError() != hipSuccess) st = ptmf_set_error(PTMF_ERR_DEVICE, "project_points: launch");
+  }
+  if (st == PTMF_OK) {
+    ptmf::k_divide<<<ptmf::nblocks(b.nnodes), ptmf::kBS, 0, s>>>(b.nnodes, A, B, A);
+    ptmf::k_q1_to_qp<<<ptmf::nblocks(27 * b.nel), ptmf::kBS, 0, s>>>(b.mx, b.my, b.mz, A, eta);
+    if (hipGetLastError() != hipSuccess) st = ptmf_set_error(PTMF_ERR_DEVICE, "project_points: launch");
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && st == PTMF_OK)
+    st = ptmf_set_error(PTMF_ERR_DEVICE, "project_points: synchronize");
+  (void)hipFree(A);
+  return st;
 }
 
 }  // extern "C"

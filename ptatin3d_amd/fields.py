@@ -296,3 +296,72 @@ def config5(mx=192, my=192, mz=96, seed=SEED):
     f = rift_body_force(coords, mx, my, mz)
     return dict(mx=mx, my=my, mz=mz, coords=coords, B=B, u0=u0, bc=bc, g=g, f=f, nexp=3.0,
                 eta_min=1e-3, eta_max=1e3)
+
+
+# ---- the reference's viscous_sinker operator test (8 x 9 x 11) ---------------
+
+def gaussian_bump_coords(mx, my, mz, amp=-0.3, lam=-5.6) -> np.ndarray:
+    """Unit box whose top surface (y = 1) is lowered by a Gaussian bump:
+    on [-1,1]^3, y_j = -1 + j (h(x,z) + 1) / (Y - 1) with h = amp exp(lam (x^2 +
+    z^2)) + 1, then mapped back to [0,1]^3 (MeshDeformation_GaussianBump_YMAX,
+    mesh_deformation.c:48-130, as the viscous_sinker model applies it,
+    model_viscous_sinker_ops.c:668-680)."""
+    X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+    xs = -1.0 + np.arange(X) * (2.0 / (X - 1))
+    zs = -1.0 + np.arange(Z) * (2.0 / (Z - 1))
+    xn, zn = np.meshgrid(xs, zs)  # (Z, X)
+    dy = (amp * np.exp(lam * (xn * xn + zn * zn)) + 1.0 + 1.0) / (Y - 1)
+    yn = -1.0 + np.arange(Y)[None, :, None] * dy[:, None, :]  # (Z, Y, X)
+    c = np.empty((Z, Y, X, 3))
+    c[..., 0] = ((xn + 1.0) / 2.0)[:, None, :]
+    c[..., 1] = (yn + 1.0) / 2.0
+    c[..., 2] = ((zn + 1.0) / 2.0)[:, None, :]
+    return c.reshape(-1)
+
+
+def _q2_1d(t):
+    return np.stack([0.5 * t * (t - 1.0), (1.0 + t) * (1.0 - t), 0.5 * (1.0 + t) * t], axis=-1)
+
+
+def sinker_markers(coords, mx, my, mz, eta0=1.0, eta1=10.0, nxp=2):
+    """Material points of the viscous_sinker model: an unperturbed nxp^3
+    lattice per element at xi = -1 + (2/nxp)(i + 1/2)
+    (SwarmMPntStd_CoordAssignment_LatticeLayout3d, material_point_std_utils.c:
+    100-210), viscosity eta1 inside the sphere of radius 0.25 about the box
+    centre and eta0 outside (single spherical inclusion,
+    model_viscous_sinker_ops.c:715-790).  Returns (element, xi [n,3], eta)."""
+    g = -1.0 + (2.0 / nxp) * (np.arange(nxp) + 0.5)
+    xi = np.array([[g[a], g[b], g[c]] for c in range(nxp) for b in range(nxp) for a in range(nxp)])
+    npc = xi.shape[0]
+    N = np.einsum("pi,pj,pk->pkji", _q2_1d(xi[:, 0]), _q2_1d(xi[:, 1]), _q2_1d(xi[:, 2])).reshape(npc, 27)
+    nid = element_node_ids(mx, my, mz)
+    xe = coords.reshape(-1, 3)[nid]
+    xp = np.einsum("pn,end->epd", N, xe)
+    inside = (((xp - 0.5) / 0.25) ** 2).sum(axis=-1) < 1.0
+    E = mx * my * mz
+    elem = np.repeat(np.arange(E), npc)
+    return elem, np.tile(xi, (E, 1)), np.where(inside, eta1, eta0).reshape(-1)
+
+
+def sinker_bc(mx, my, mz) -> np.ndarray:
+    """Free slip with a free top (VSBC_FreeSlipFreeSurface, the viscous_sinker
+    default, model_viscous_sinker_ops.c:236-244): u_x on the x-faces, u_y on
+    the base y = 0, u_z on the z-faces; 3N mask."""
+    X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+    m = np.zeros((Z, Y, X, 3), dtype=np.uint8)
+    m[:, :, 0, 0] = m[:, :, -1, 0] = 1
+    m[:, 0, :, 1] = 1
+    m[0, :, :, 2] = m[-1, :, :, 2] = 1
+    return m.reshape(-1)
+
+
+def reference_pressure_test_vector(mx, my, mz) -> np.ndarray:
+    """p[4e+i] = X + Y + Z + i + 3 with X = -1 + ei 2/(mx-1) etc.
+    (_GenerateTestVectorDAP, test_stokes_operators.c:94-136)."""
+    ei, ej, ek = np.meshgrid(np.arange(mx), np.arange(my), np.arange(mz), indexing="ij")
+    e = (ei + mx * (ej + my * ek)).ravel()
+    base = (-1.0 + ei * 2.0 / (mx - 1) - 1.0 + ej * 2.0 / (my - 1) - 1.0 + ek * 2.0 / (mz - 1)).ravel()
+    p = np.zeros(4 * mx * my * mz)
+    for i in range(4):
+        p[4 * e + i] = base + i + 3.0
+    return p

@@ -103,6 +103,10 @@ class MatA11MF:
         call("ptmf_op_powerlaw_viscosity", self.handle, u.ptr, B.ptr, float(nexp), float(eta_min),
              float(eta_max), eta.ptr)
 
+    def project_points(self, elem: DeviceArray, xi: DeviceArray, val: DeviceArray, eta: DeviceArray) -> None:
+        """Material-point values to the quadrature points (element-major 27E)."""
+        call("ptmf_op_project_points", self.handle, int(val.n), elem.ptr, xi.ptr, val.ptr, eta.ptr)
+
     def set_viscosity_device(self, eta: DeviceArray) -> None:
         call("ptmf_op_set_viscosity_device", self.handle, eta.ptr)
 

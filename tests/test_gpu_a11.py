@@ -203,3 +203,42 @@ def test_coefficient_state_bumps(gpu):
     free = bc == 0
     assert rel_inf(y1[free], 2.0 * y0[free]) <= TOL64
     A.destroy()
+
+
+def _close9(v, ref):
+    """Equal to the 9 significant digits the reference prints."""
+    return abs(v - ref) <= 0.5 * 10.0 ** (np.floor(np.log10(abs(ref))) - 8)
+
+
+def test_reference_golden_values_viscous_sinker(gpu):
+    """The reference's operator test (viscous_sinker 8x9x11, Gaussian-bump
+    mesh, lattice markers projected on the GPU, free slip with a free top):
+    the GPU's A11, diag(A11), A21 and A12 reproduce the printed y.y values of
+    compare_mf_8x9x11/expected."""
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "viscous_sinker_8x9x11.json")))
+    m = tuple(g["mesh"])
+    E = m[0] * m[1] * m[2]
+    coords = F.gaussian_bump_coords(*m)
+    elem, xi, val = F.sinker_markers(coords, *m)
+    bc = F.sinker_bc(*m)
+    A = MatStokesMF(*m).setup(coords, np.ones(27 * E), bc)
+    de = DeviceArray(27 * E)
+    A.project_points(DeviceArray.from_host(elem.astype(np.int64)), DeviceArray.from_host(xi.reshape(-1)),
+                     DeviceArray.from_host(val), de)
+    eta = O.q1_to_qp(*m, O.q1_nodal_from_points(*m, elem, xi, val))
+    assert rel_inf(de.download(), eta) <= 1e-14
+    A.set_viscosity_device(de)
+    xu = F.reference_test_vector(coords)
+    xp = F.reference_pressure_test_vector(*m)
+    y = A.mult_host(xu)
+    assert _close9(y @ y, g["A11_y.y"]), y @ y
+    assert rel_inf(y, O.a11_matmult(*m, coords, eta, bc, xu)) <= TOL64
+    d = A.diagonal_host()
+    assert _close9(d @ d, g["diagA11_y.y"]), d @ d
+    y = A.a21_host(xu)
+    assert _close9(y @ y, g["A21_y.y"]), y @ y
+    y = A.a12_host(xp)
+    assert _close9(y @ y, g["A12_y.y"]), y @ y
+    A.destroy()
