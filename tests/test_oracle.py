@@ -408,15 +408,11 @@ def test_q1_hierarchy_against_numpy(inject):
         assert np.abs(g - 3.25).max() <= 1e-14
 
 
-def _sinker_case():
+def _golden_cases():
     import json
     import os
-    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "viscous_sinker_8x9x11.json")))
-    m = tuple(g["mesh"])
-    coords = F.gaussian_bump_coords(*m)
-    elem, xi, val = F.sinker_markers(coords, *m)
-    eta = O.q1_to_qp(*m, O.q1_nodal_from_points(*m, elem, xi, val))
-    return g, m, coords, eta, F.sinker_bc(*m)
+    path = os.path.join(os.path.dirname(__file__), "golden", "viscous_sinker_operators.json")
+    return json.load(open(path))["cases"]
 
 
 def _close9(v, ref):
@@ -425,24 +421,32 @@ def _close9(v, ref):
     return abs(v - ref) <= 0.5 * 10.0 ** (np.floor(np.log10(abs(ref))) - 8)
 
 
-def test_reference_golden_values_viscous_sinker():
-    """The reference's own operator test (viscous_sinker, 8x9x11, Gaussian-bump
+@pytest.mark.parametrize("case", _golden_cases(), ids=lambda c: "x".join(map(str, c["mesh"])) + f"_eta{c['eta1']:g}")
+def test_reference_golden_values_viscous_sinker(case):
+    """The reference's own operator test (viscous_sinker on the Gaussian-bump
     mesh, 2x2x2 lattice markers projected to the quadrature points, free slip
     with a free top): y.y of A11 x, A21 x, A12 p and diag(A11) equal the
     printed expected values (test_stokes_operators.c:225-580)."""
-    g, m, coords, eta, bc = _sinker_case()
+    m = tuple(case["mesh"])
+    coords = F.gaussian_bump_coords(*m)
+    elem, xi, val = F.sinker_markers(coords, *m, eta1=case["eta1"])
+    eta = O.q1_to_qp(*m, O.q1_nodal_from_points(*m, elem, xi, val))
+    bc = F.sinker_bc(*m)
     xu = F.reference_test_vector(coords)
-    xp = F.reference_pressure_test_vector(*m)
     y = O.a11_matmult(*m, coords, eta, bc, xu)
-    assert _close9(y @ y, g["A11_y.y"]), y @ y
+    assert _close9(y @ y, case["A11"]), y @ y
     y = O.a21_matmult(*m, coords, bc, xu)
-    assert _close9(y @ y, g["A21_y.y"]), y @ y
-    y = O.a12_matmult(*m, coords, bc, xp)
-    assert _close9(y @ y, g["A12_y.y"]), y @ y
+    assert _close9(y @ y, case["A21"]), y @ y
+    if case["A12"] is not None:
+        y = O.a12_matmult(*m, coords, bc, F.reference_pressure_test_vector(*m))
+        assert _close9(y @ y, case["A12"]), y @ y
     d = O.a11_diagonal(*m, coords, eta, bc)
-    assert _close9(d @ d, g["diagA11_y.y"]), d @ d
-    # the marker projection with the points placed at the quadrature points
-    # is the quadrature-point projection
+    assert _close9(d @ d, case["diagA11"]), d @ d
+
+
+def test_point_projection_at_quadrature_points_is_qp_projection():
+    m = (3, 2, 4)
+    eta = F.eta_loguniform(m[0] * m[1] * m[2], 1e-2, 1e2, np.random.default_rng(2))
     qx = np.tile(np.stack(np.meshgrid(F.GAUSS, F.GAUSS, F.GAUSS, indexing="ij")[::-1], -1).reshape(27, 3),
                  (m[0] * m[1] * m[2], 1))
     el = np.repeat(np.arange(m[0] * m[1] * m[2]), 27)
