@@ -53,6 +53,7 @@ extern "C" {
 typedef struct ptmf_op_s *ptmf_op;
 typedef struct ptmf_mg_s *ptmf_mg;
 typedef struct ptmf_batch_s *ptmf_batch;
+typedef struct ptmf_bmg_s *ptmf_bmg;
 
 /* Options of the geometric-multigrid preconditioner of the viscous block
  * (the reference configures the same solver through PETSc options:
@@ -271,6 +272,24 @@ int ptmf_batch_mult(ptmf_batch batch, int precision, const void *const *x, void 
 /* ms per batched MatMult over reps back-to-back launches (HIP events) */
 int ptmf_batch_time(ptmf_batch batch, int precision, const void *const *x, void *const *y, int reps,
                     double *ms_per_mult);
+
+/* ---- batched multigrid: many independent boxes in lockstep --------------- *
+ * No counterpart in the reference (one solve per PETSc communicator there):
+ * the velocity-block solves of many independent boxes, each a set-up ptmf_mg
+ * (any box sizes; same number of levels and smoother options; lattice
+ * transfers, rediscretised levels, point-Jacobi Chebyshev, dense coarse
+ * inverse), run with one launch per stage over all boxes and the V-cycle
+ * captured once as a HIP graph.  The ptmf_mg objects must outlive the batch
+ * and must not be used while it runs (their level buffers are shared). */
+int ptmf_bmg_create(const ptmf_mg *mgs, int nbox, ptmf_bmg *out);
+int ptmf_bmg_destroy(ptmf_bmg *bmg);
+/* x[b] = V-cycle(b[b]) for every box (host arrays of device pointers). */
+int ptmf_bmg_vcycle(ptmf_bmg bmg, const double *const *b, double *const *x);
+/* Flexible CG (FP64) on every box from the given x[b], preconditioned by the
+ * batched V-cycle; box b stops once ||r_b|| <= rtol ||b_b||.  its[b] and
+ * rnorm[b] (the true residual at return) are host arrays of nbox entries. */
+int ptmf_bmg_fcg(ptmf_bmg bmg, const double *const *b, double *const *x, double rtol, int maxit, int *its,
+                 double *rnorm);
 
 /* ---- saddle point [A11 A12; A21 0] on the finest level ------------------ *
  * Vectors are [u (3N) | p (4E)] device arrays.  The solver is FGMRES with the

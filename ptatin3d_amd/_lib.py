@@ -89,6 +89,10 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_batch_destroy": (_I, [_P]),
     "ptmf_batch_mult": (_I, [_P, _I, _P, _P]),
     "ptmf_batch_time": (_I, [_P, _I, _P, _P, _I, _PD]),
+    "ptmf_bmg_create": (_I, [_P, _I, _P]),
+    "ptmf_bmg_destroy": (_I, [_P]),
+    "ptmf_bmg_vcycle": (_I, [_P, _P, _P]),
+    "ptmf_bmg_fcg": (_I, [_P, _P, _P, _D, _I, _P, _P]),
     "ptmf_mg_stokes_setup": (_I, [_P, _I, _D]),
     "ptmf_mg_stokes_apply": (_I, [_P, _P, _P]),
     "ptmf_mg_stokes_fgmres": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _PD, _PD, _I]),

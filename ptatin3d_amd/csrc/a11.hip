@@ -1309,8 +1309,9 @@ struct ptmf_batch_s {
 
 namespace {
 
+// work list and per-box descriptors of precision T (no vectors)
 template <typename T>
-int batch_prepare(ptmf_batch_s *b, const void *const *x, void *const *y) {
+int batch_prepare_boxes(ptmf_batch_s *b) {
   const int ip = sizeof(T) == 8 ? 0 : 1;
   const int n = (int)b->ops.size();
   // work list of (box, tile)
@@ -1353,6 +1354,8 @@ int batch_prepare(ptmf_batch_s *b, const void *const *x, void *const *y) {
     d.fbz = (T *)d.fby + 3 * d.a.fby_n * d.a.ntiles;
     if (op->nbc > b->maxnbc) b->maxnbc = op->nbc;
   }
+  // the tile-ordered viscosity is built on each operator's own stream
+  for (ptmf_op op : b->ops) PTMF_HIP(hipStreamSynchronize(op->stream));
   const size_t bytes = sizeof(ptmf::BoxDesc) * n;
   if (!b->dbox[ip]) PTMF_HIP(hipMalloc((void **)&b->dbox[ip], bytes));
   const unsigned char *raw = (const unsigned char *)box.data();
@@ -1360,6 +1363,14 @@ int batch_prepare(ptmf_batch_s *b, const void *const *x, void *const *y) {
     b->hbox[ip].assign(raw, raw + bytes);
     PTMF_HIP(hipMemcpyAsync(b->dbox[ip], b->hbox[ip].data(), bytes, hipMemcpyHostToDevice, b->stream));
   }
+  return PTMF_OK;
+}
+
+template <typename T>
+int batch_prepare(ptmf_batch_s *b, const void *const *x, void *const *y) {
+  const int ip = sizeof(T) == 8 ? 0 : 1;
+  const int n = (int)b->ops.size();
+  PTMF_TRY(batch_prepare_boxes<T>(b));
   // x and y pointer arrays
   std::vector<const void *> xy(2 * n);
   for (int i = 0; i < n; i++) {
@@ -1376,28 +1387,48 @@ int batch_prepare(ptmf_batch_s *b, const void *const *x, void *const *y) {
   return PTMF_OK;
 }
 
+// the three batched stages on device tables of x and y pointers
 template <typename T>
-int batch_launch(ptmf_batch_s *b) {
+int batch_launch_tables(ptmf_batch_s *b, const T *const *xs, T *const *ys, hipStream_t s) {
   const int ip = sizeof(T) == 8 ? 0 : 1;
   const int n = (int)b->ops.size();
-  const T *const *xs = (const T *const *)b->dxy[ip];
-  T *const *ys = (T *const *)(b->dxy[ip] + n);
-  ptmf::a11_tile_batch_kernel<T><<<dim3((unsigned)b->nwork), dim3(64), 0, b->stream>>>(b->dbox[ip], b->dwork,
-                                                                                      b->nwork, xs, (T *const *)ys);
+  ptmf::a11_tile_batch_kernel<T><<<dim3((unsigned)b->nwork), dim3(64), 0, s>>>(b->dbox[ip], b->dwork, b->nwork, xs,
+                                                                              ys);
   PTMF_HIP(hipGetLastError());
-  ptmf::a11_face_batch_kernel<T><<<dim3((unsigned)b->nwork), dim3(64), 0, b->stream>>>(b->dbox[ip], b->dwork,
-                                                                                       b->nwork, ys);
+  ptmf::a11_face_batch_kernel<T><<<dim3((unsigned)b->nwork), dim3(64), 0, s>>>(b->dbox[ip], b->dwork, b->nwork, ys);
   PTMF_HIP(hipGetLastError());
   if (b->maxnbc > 0) {
     long gx = (b->maxnbc + 255) / 256;
     if (gx > 1024) gx = 1024;
-    ptmf::bc_copy_batch_kernel<T><<<dim3((unsigned)gx, (unsigned)n), dim3(256), 0, b->stream>>>(b->dbox[ip], xs, ys);
+    ptmf::bc_copy_batch_kernel<T><<<dim3((unsigned)gx, (unsigned)n), dim3(256), 0, s>>>(b->dbox[ip], xs, ys);
     PTMF_HIP(hipGetLastError());
   }
   return PTMF_OK;
 }
 
+template <typename T>
+int batch_launch(ptmf_batch_s *b) {
+  const int ip = sizeof(T) == 8 ? 0 : 1;
+  const int n = (int)b->ops.size();
+  return batch_launch_tables<T>(b, (const T *const *)b->dxy[ip], (T *const *)(b->dxy[ip] + n), b->stream);
+}
+
 }  // namespace
+
+namespace ptmf {
+int batch_bind(ptmf_batch_s *b, int precision) {
+  for (ptmf_op op : b->ops) PTMF_HIP(hipStreamSynchronize(op->stream));
+  if (precision == 64) PTMF_TRY(batch_prepare_boxes<double>(b));
+  else PTMF_TRY(batch_prepare_boxes<float>(b));
+  PTMF_HIP(hipStreamSynchronize(b->stream));
+  return PTMF_OK;
+}
+
+int batch_apply(ptmf_batch_s *b, int precision, const void *const *xs, void *const *ys, hipStream_t s) {
+  if (precision == 64) return batch_launch_tables<double>(b, (const double *const *)xs, (double *const *)ys, s);
+  return batch_launch_tables<float>(b, (const float *const *)xs, (float *const *)ys, s);
+}
+}  // namespace ptmf
 
 extern "C" {
 

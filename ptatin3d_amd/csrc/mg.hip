@@ -1903,3 +1903,671 @@ int ptmf_op_project_points(ptmf_op op, long npts, const long *elem, const double
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batched multigrid: many independent boxes (each a set-up ptmf_mg with its
+// own sizes, coefficients, eigenvalue bounds and dense coarse inverse) run
+// the V-cycle and flexible CG in lockstep.  Every stage is one launch over
+// all boxes: the level MatMults through the batched tile kernels, the vector
+// stages on a (chunk, box) grid over per-box pointer tables, the transfers
+// with per-box lattice sizes and the coarse solve as one dense product per
+// box.  The V-cycle is captured once as a HIP graph.
+// ===========================================================================
+struct ptmf_bmg_s;
+
+namespace ptmf {
+namespace {
+
+// per-level vector roles of the pointer tables (P64 and P32 take three slots)
+enum BRole {
+  kB64 = 0, kX64 = 1, kR64 = 2, kP64a = 3, kD64 = 6,
+  kB32 = 7, kX32 = 8, kR32 = 9, kP32a = 10, kD32 = 13, kNRoles = 14
+};
+constexpr int kSegBlocks = 32;  // blocks per box of the vector stages
+constexpr int kMaxChebIts = 64;
+
+template <typename T>
+__global__ void kb_fill(const long *__restrict__ n, T *const *y, T v) {
+  const int b = blockIdx.y;
+  T *yb = y[b];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x) yb[i] = v;
+}
+
+template <typename Ts, typename Td>
+__global__ void kb_cast(const long *__restrict__ n, const Ts *const *x, Td *const *y) {
+  const int b = blockIdx.y;
+  const Ts *xb = x[b];
+  Td *yb = y[b];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x)
+    yb[i] = (Td)xb[i];
+}
+
+template <typename T>
+__global__ void kb_residual(const long *__restrict__ n, const T *const *bb, const T *const *Ax, T *const *r) {
+  const int b = blockIdx.y;
+  const T *bv = bb[b], *av = Ax[b];
+  T *rv = r[b];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x)
+    rv[i] = bv[i] - av[i];
+}
+
+// first Chebyshev step, per-box scale: p = x + scale dinv (b - Ax) (x, Ax may be absent)
+template <typename T>
+__global__ void kb_cheb_first(const long *__restrict__ n, const T *const *bb, const T *const *Ax, const T *const *x,
+                              const T *const *dinv, const double *__restrict__ scale, T *const *p) {
+  const int b = blockIdx.y;
+  const T *bv = bb[b], *av = Ax ? Ax[b] : nullptr, *xv = x ? x[b] : nullptr, *dv = dinv[b];
+  T *pv = p[b];
+  const T sc = (T)scale[b];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x) {
+    const T r = av ? bv[i] - av[i] : bv[i];
+    const T x0 = xv ? xv[i] : T(0);
+    pv[i] = x0 + sc * (r * dv[i]);
+  }
+}
+
+// p_kp1 = c0 dinv (b - A p_k) + c1 p_km1 + c2 p_k with per-box (c0, c1, c2)
+template <typename T>
+__global__ void kb_cheb_step(const long *__restrict__ n, const T *const *bb, const T *const *Apk, const T *const *dinv,
+                             const T *const *pkm1, const T *const *pk, const double *__restrict__ coef,
+                             T *const *pkp1) {
+  const int b = blockIdx.y;
+  const T *bv = bb[b], *av = Apk[b], *dv = dinv[b], *pm = pkm1[b], *pc = pk[b];
+  T *pn = pkp1[b];
+  const T c0 = (T)coef[3 * b], c1 = (T)coef[3 * b + 1], c2 = (T)coef[3 * b + 2];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x)
+    pn[i] = c0 * ((bv[i] - av[i]) * dv[i]) + c1 * pm[i] + c2 * pc[i];
+}
+
+template <typename T>
+__global__ void kb_bc_set(const long *__restrict__ nbc, const long *const *idx, T v, T *const *y) {
+  const int b = blockIdx.y;
+  const long *ib = idx[b];
+  T *yb = y[b];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nbc[b]; i += (long)gridDim.x * blockDim.x)
+    yb[ib[i]] = v;
+}
+
+// per-box lattice sizes: dims[6 b + (0..5)] = Xc, Yc, Zc, Xf, Yf, Zf
+template <typename T>
+__global__ void kb_restrict(const long *__restrict__ dims, const T *const *xf, T *const *yc) {
+  const int b = blockIdx.y;
+  const long *d = dims + 6 * b;
+  const long Xc = d[0], Yc = d[1], Zc = d[2], Xf = d[3], Yf = d[4], Zf = d[5];
+  const T *xv = xf[b];
+  T *yv = yc[b];
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < Xc * Yc * Zc; g += (long)gridDim.x * blockDim.x) {
+    const long i = g % Xc, j = (g / Xc) % Yc, k = g / (Xc * Yc);
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int dk = -1; dk <= 1; dk++) {
+      const long K = 2 * k + dk;
+      if (K < 0 || K >= Zf) continue;
+      const T wk = dk == 0 ? T(1) : T(0.5);
+      for (int dj = -1; dj <= 1; dj++) {
+        const long J = 2 * j + dj;
+        if (J < 0 || J >= Yf) continue;
+        const T wj = dj == 0 ? T(1) : T(0.5);
+        for (int di = -1; di <= 1; di++) {
+          const long I = 2 * i + di;
+          if (I < 0 || I >= Xf) continue;
+          const T w = (di == 0 ? T(1) : T(0.5)) * wj * wk;
+          const long f = I + Xf * (J + Yf * K);
+          s0 += w * xv[3 * f];
+          s1 += w * xv[3 * f + 1];
+          s2 += w * xv[3 * f + 2];
+        }
+      }
+    }
+    yv[3 * g] = s0;
+    yv[3 * g + 1] = s1;
+    yv[3 * g + 2] = s2;
+  }
+}
+
+template <typename T>
+__global__ void kb_prolong_add(const long *__restrict__ dims, const T *const *xc, T *const *yf) {
+  const int b = blockIdx.y;
+  const long *d = dims + 6 * b;
+  const long Xc = d[0], Yc = d[1], Xf = d[3], Yf = d[4], Zf = d[5];
+  const T *xv = xc[b];
+  T *yv = yf[b];
+  for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < Xf * Yf * Zf; f += (long)gridDim.x * blockDim.x) {
+    const long I = f % Xf, J = (f / Xf) % Yf, K = f / (Xf * Yf);
+    const long ci0 = I >> 1, cj0 = J >> 1, ck0 = K >> 1;
+    const int ni = (I & 1) ? 2 : 1, nj = (J & 1) ? 2 : 1, nk = (K & 1) ? 2 : 1;
+    const T wi = ni == 2 ? T(0.5) : T(1), wj = nj == 2 ? T(0.5) : T(1), wk = nk == 2 ? T(0.5) : T(1);
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int c = 0; c < nk; c++)
+      for (int bb = 0; bb < nj; bb++)
+        for (int a = 0; a < ni; a++) {
+          const long g = (ci0 + a) + Xc * ((cj0 + bb) + Yc * (ck0 + c));
+          const T w = wi * wj * wk;
+          s0 += w * xv[3 * g];
+          s1 += w * xv[3 * g + 1];
+          s2 += w * xv[3 * g + 2];
+        }
+    yv[3 * f] += s0;
+    yv[3 * f + 1] += s1;
+    yv[3 * f + 2] += s2;
+  }
+}
+
+// y = A x per box with the explicit inverse's lower triangle (column-major)
+__global__ void kb_symv(const long *__restrict__ n, const double *const *A, const double *const *x, double *const *y) {
+  const int b = blockIdx.y;
+  const long nb = n[b];
+  const double *Ab = A[b], *xb = x[b];
+  double *yb = y[b];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nb; i += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (long j = 0; j <= i; j++) s += Ab[i + j * nb] * xb[j];     // A(i,j), j <= i: column j, row i
+    for (long j = i + 1; j < nb; j++) s += Ab[j + i * nb] * xb[j];  // A(i,j) = A(j,i), j > i
+    yb[i] = s;
+  }
+}
+
+// per-box partial sums of up to three dot products <x_k, y_k>: part[(b * kSegBlocks + block) * 3 + k]
+__global__ void kb_dots(const long *__restrict__ n, int nd, const double *const *x0, const double *const *y0,
+                        const double *const *x1, const double *const *y1, const double *const *x2,
+                        const double *const *y2, double *__restrict__ part) {
+  __shared__ double red[3][kBS / 64];
+  const int b = blockIdx.y;
+  const double *a0 = x0[b], *b0 = y0[b];
+  const double *a1 = nd > 1 ? x1[b] : nullptr, *b1 = nd > 1 ? y1[b] : nullptr;
+  const double *a2 = nd > 2 ? x2[b] : nullptr, *b2 = nd > 2 ? y2[b] : nullptr;
+  double s[3] = {0, 0, 0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x) {
+    s[0] += a0[i] * b0[i];
+    if (nd > 1) s[1] += a1[i] * b1[i];
+    if (nd > 2) s[2] += a2[i] * b2[i];
+  }
+  for (int k = 0; k < 3; k++) {
+    double v = s[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double v = 0.0;
+    for (int w = 0; w < kBS / 64; w++) v += red[threadIdx.x][w];
+    part[((long)b * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = v;
+  }
+}
+
+// x += a p, rold = r, r -= a q (per-box a)
+__global__ void kb_fcg_update(const long *__restrict__ n, const double *__restrict__ alpha, const double *const *p,
+                              const double *const *q, double *const *x, double *const *r, double *const *rold) {
+  const int b = blockIdx.y;
+  const double a = alpha[b];
+  const double *pv = p[b], *qv = q[b];
+  double *xv = x[b], *rv = r[b], *ov = rold[b];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x) {
+    xv[i] += a * pv[i];
+    ov[i] = rv[i];
+    rv[i] -= a * qv[i];
+  }
+}
+
+// p = z + beta p where active, per-box beta
+__global__ void kb_fcg_dir(const long *__restrict__ n, const double *__restrict__ beta, const int *__restrict__ active,
+                           const double *const *z, double *const *p) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const double be = beta[b];
+  const double *zv = z[b];
+  double *pv = p[b];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x)
+    pv[i] = zv[i] + be * pv[i];
+}
+
+struct BLevel {
+  ptmf_batch bt = nullptr;           // batched MatMult of the boxes' level operators
+  long *dn = nullptr;                // per-box vector length
+  long *dnbc = nullptr;              // per-box Dirichlet count
+  void **tab = nullptr;              // [kNRoles][nbox] vector pointers
+  const long **bcidx = nullptr;      // per-box Dirichlet lists
+  long *dims = nullptr;              // per-box lattice sizes (this level coarse, next finer fine), 6 per box
+  double *scale = nullptr;           // per-box Chebyshev scale
+  double *coef = nullptr;            // [kMaxChebIts][nbox][3] Chebyshev step coefficients
+};
+
+}  // namespace
+}  // namespace ptmf
+
+struct ptmf_bmg_s {
+  std::vector<ptmf_mg> mgs;
+  int L = 0, nbox = 0, prec = 32, pre = 0, post = 0, bc_transfer = 1;
+  hipStream_t stream = nullptr;
+  ptmf::BLevel lev[16];
+  // coarse dense inverses
+  const double **dA = nullptr;
+  long *dn0 = nullptr;
+  // flexible CG on the finest level: r, p, q, rold per box (one allocation) and their tables
+  double *fbuf = nullptr;
+  void **ftab = nullptr;  // [7][nbox]: b, x, r, p, q, rold, z
+  double *part = nullptr, *hpart = nullptr;
+  double *dscal = nullptr;  // [2][nbox]: alpha, beta
+  int *dactive = nullptr;
+  // captured V-cycle (finest B64 -> finest X64)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int vcalls = 0;
+};
+
+namespace ptmf {
+namespace {
+
+inline dim3 seg_grid(int nbox) { return dim3(kSegBlocks, (unsigned)nbox); }
+
+template <typename T>
+T *const *btab(const BLevel &l, int role, int nbox) {
+  return (T *const *)(l.tab + (long)role * nbox);
+}
+
+int bmatmult(ptmf_bmg_s *m, int l, int prec, int src, int dst) {
+  const BLevel &B = m->lev[l];
+  return batch_apply(B.bt, prec, (const void *const *)(B.tab + (long)src * m->nbox),
+                     (void *const *)(B.tab + (long)dst * m->nbox), m->stream);
+}
+
+template <typename T>
+int bcheb(ptmf_bmg_s *m, int l, int rb, int rx, int its, bool nonzero) {
+  const BLevel &B = m->lev[l];
+  const int nb = m->nbox, w = sizeof(T) == 8 ? 64 : 32;
+  const int rp = sizeof(T) == 8 ? kP64a : kP32a, rr = sizeof(T) == 8 ? kR64 : kR32, rd = sizeof(T) == 8 ? kD64 : kD32;
+  int km1 = 0, k = 1, kp1 = 2;
+  T *const *P[3] = {btab<T>(B, rp, nb), btab<T>(B, rp + 1, nb), btab<T>(B, rp + 2, nb)};
+  const int Prole[3] = {rp, rp + 1, rp + 2};
+  hipStream_t s = m->stream;
+  if (nonzero) {
+    kb_cast<T, T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rx, nb), P[km1]);
+    PTMF_TRY(bmatmult(m, l, w, rx, rr));
+    kb_cheb_first<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rb, nb), btab<T>(B, rr, nb), btab<T>(B, rx, nb),
+                                                  btab<T>(B, rd, nb), B.scale, P[k]);
+  } else {
+    kb_fill<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, P[km1], T(0));
+    kb_cheb_first<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rb, nb), nullptr, nullptr, btab<T>(B, rd, nb),
+                                                  B.scale, P[k]);
+  }
+  PTMF_HIP(hipGetLastError());
+  for (int i = 0; i < its; i++) {
+    PTMF_TRY(bmatmult(m, l, w, Prole[k], rr));
+    kb_cheb_step<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rb, nb), btab<T>(B, rr, nb), btab<T>(B, rd, nb),
+                                                 P[km1], P[k], B.coef + (long)i * 3 * nb, P[kp1]);
+    PTMF_HIP(hipGetLastError());
+    const int t = km1;
+    km1 = k; k = kp1; kp1 = t;
+  }
+  kb_cast<T, T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, P[k], btab<T>(B, rx, nb));
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+// V-cycle from level l (right-hand side role rb, result role rx) down to the
+// dense coarse solve on level 0; zero initial guess
+template <typename T>
+int bvcycle(ptmf_bmg_s *m, int l, int rb, int rx) {
+  const int nb = m->nbox, w = sizeof(T) == 8 ? 64 : 32;
+  const BLevel &B = m->lev[l];
+  hipStream_t s = m->stream;
+  if (l == 0) {
+    if (sizeof(T) == 4) kb_cast<float, double><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<float>(B, rb, nb),
+                                                                           btab<double>(B, kB64, nb));
+    kb_symv<<<seg_grid(nb), kBS, 0, s>>>(m->dn0, m->dA, btab<double>(B, sizeof(T) == 4 ? kB64 : rb, nb),
+                                         btab<double>(B, sizeof(T) == 4 ? kX64 : rx, nb));
+    if (sizeof(T) == 4) kb_cast<double, float><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<double>(B, kX64, nb),
+                                                                           btab<float>(B, rx, nb));
+    PTMF_HIP(hipGetLastError());
+    return PTMF_OK;
+  }
+  const BLevel &C = m->lev[l - 1];
+  const int rr = sizeof(T) == 8 ? kR64 : kR32, cb = sizeof(T) == 8 ? kB64 : kB32, cx = sizeof(T) == 8 ? kX64 : kX32;
+  PTMF_TRY(bcheb<T>(m, l, rb, rx, m->pre, false));
+  PTMF_TRY(bmatmult(m, l, w, rx, rr));
+  kb_residual<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rb, nb), btab<T>(B, rr, nb), btab<T>(B, rr, nb));
+  kb_restrict<T><<<seg_grid(nb), kBS, 0, s>>>(B.dims, btab<T>(B, rr, nb), btab<T>(C, cb, nb));
+  if (m->bc_transfer) kb_bc_set<T><<<seg_grid(nb), kBS, 0, s>>>(C.dnbc, C.bcidx, T(0), btab<T>(C, cb, nb));
+  PTMF_HIP(hipGetLastError());
+  PTMF_TRY(bvcycle<T>(m, l - 1, cb, cx));
+  kb_prolong_add<T><<<seg_grid(nb), kBS, 0, s>>>(B.dims, btab<T>(C, cx, nb), btab<T>(B, rx, nb));
+  PTMF_HIP(hipGetLastError());
+  return bcheb<T>(m, l, rb, rx, m->post, true);
+}
+
+// finest B64 -> finest X64
+int bvcycle_body(ptmf_bmg_s *m) {
+  const int lf = m->L - 1, nb = m->nbox;
+  const BLevel &F = m->lev[lf];
+  if (m->prec == 64) return bvcycle<double>(m, lf, kB64, kX64);
+  kb_cast<double, float><<<seg_grid(nb), kBS, 0, m->stream>>>(F.dn, btab<double>(F, kB64, nb), btab<float>(F, kB32, nb));
+  PTMF_TRY(bvcycle<float>(m, lf, kB32, kX32));
+  kb_cast<float, double><<<seg_grid(nb), kBS, 0, m->stream>>>(F.dn, btab<float>(F, kX32, nb), btab<double>(F, kX64, nb));
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+int bvcycle_run(ptmf_bmg_s *m) {
+  if (m->vcalls++ == 0) return bvcycle_body(m);
+  if (!m->exec) {
+    PTMF_HIP(hipStreamSynchronize(m->stream));
+    PTMF_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    const int st = bvcycle_body(m);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(m->stream, &g);
+    PTMF_TRY(st);
+    PTMF_CHECK(ce == hipSuccess && g, PTMF_ERR_DEVICE, "bmg: graph capture failed: %s", hipGetErrorString(ce));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return ptmf_set_error(PTMF_ERR_DEVICE, "bmg: graph instantiation failed: %s", hipGetErrorString(ie));
+    }
+    m->graph = g;
+    m->exec = ex;
+  }
+  PTMF_HIP(hipGraphLaunch(m->exec, m->stream));
+  return PTMF_OK;
+}
+
+// per-box sums of the partial dot products (one host synchronisation)
+int bdots(ptmf_bmg_s *m, int nd, const void *const *x0, const void *const *y0, const void *const *x1,
+          const void *const *y1, const void *const *x2, const void *const *y2, std::vector<double> &out) {
+  const int nb = m->nbox;
+  const BLevel &F = m->lev[m->L - 1];
+  kb_dots<<<seg_grid(nb), kBS, 0, m->stream>>>(F.dn, nd, (const double *const *)x0, (const double *const *)y0,
+                                                (const double *const *)x1, (const double *const *)y1,
+                                                (const double *const *)x2, (const double *const *)y2, m->part);
+  PTMF_HIP(hipGetLastError());
+  const size_t np = (size_t)nb * kSegBlocks * 3;
+  PTMF_HIP(hipMemcpyAsync(m->hpart, m->part, sizeof(double) * np, hipMemcpyDeviceToHost, m->stream));
+  PTMF_HIP(hipStreamSynchronize(m->stream));
+  out.assign((size_t)nb * 3, 0.0);
+  for (int b = 0; b < nb; b++)
+    for (int c = 0; c < kSegBlocks; c++)
+      for (int k = 0; k < 3; k++) out[3 * b + k] += m->hpart[((size_t)b * kSegBlocks + c) * 3 + k];
+  return PTMF_OK;
+}
+
+}  // namespace
+}  // namespace ptmf
+
+extern "C" {
+
+int ptmf_bmg_destroy(ptmf_bmg *pm) {
+  if (!pm || !*pm) return PTMF_OK;
+  ptmf_bmg_s *m = *pm;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  if (m->exec) (void)hipGraphExecDestroy(m->exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  for (int l = 0; l < m->L; l++) {
+    ptmf::BLevel &B = m->lev[l];
+    if (B.bt) (void)ptmf_batch_destroy(&B.bt);
+    void *ps[] = {B.dn, B.dnbc, B.tab, B.bcidx, B.dims, B.scale, B.coef};
+    for (void *p : ps) (void)hipFree(p);
+  }
+  void *ps[] = {m->dA, m->dn0, m->fbuf, m->ftab, m->part, m->dscal, m->dactive};
+  for (void *p : ps) (void)hipFree(p);
+  if (m->hpart) (void)hipHostFree(m->hpart);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  *pm = nullptr;
+  return PTMF_OK;
+}
+
+int ptmf_bmg_create(const ptmf_mg *mgs, int nbox, ptmf_bmg *out) {
+  PTMF_CHECK(mgs && out && nbox > 0 && nbox <= 65535, PTMF_ERR_ARG, "bmg_create: bad arguments");
+  *out = nullptr;
+  const ptmf_mg m0 = mgs[0];
+  PTMF_CHECK(m0, PTMF_ERR_ARG, "bmg_create: null multigrid 0");
+  for (int b = 0; b < nbox; b++) {
+    const ptmf_mg g = mgs[b];
+    PTMF_CHECK(g && g->is_setup, PTMF_ERR_NOT_SETUP, "bmg_create: multigrid %d not set up", b);
+    const ptmf_mg_options &o = g->opt;
+    PTMF_CHECK(g->L == m0->L && o.smoother_precision == m0->opt.smoother_precision && o.pre_its == m0->opt.pre_its &&
+                   o.post_its == m0->opt.post_its && o.bc_transfer == m0->opt.bc_transfer,
+               PTMF_ERR_ARG, "bmg_create: multigrid %d: levels and smoother options must match box 0", b);
+    PTMF_CHECK(o.transfer == 0 && o.galerkin_levels == 0 && o.levels_mg == g->L && o.smoother_diag == 0,
+               PTMF_ERR_UNSUPPORTED,
+               "bmg_create: multigrid %d: needs lattice transfers, rediscretised levels, point Jacobi and the "
+               "dense coarse solve", b);
+    PTMF_CHECK(o.pre_its <= ptmf::kMaxChebIts && o.post_its <= ptmf::kMaxChebIts && g->L >= 2, PTMF_ERR_UNSUPPORTED,
+               "bmg_create: at most %d smoothing steps and at least 2 levels", ptmf::kMaxChebIts);
+    PTMF_HIP(hipStreamSynchronize(g->stream));
+  }
+  ptmf_bmg_s *m = new ptmf_bmg_s;
+  m->mgs.assign(mgs, mgs + nbox);
+  m->L = m0->L;
+  m->nbox = nbox;
+  m->prec = m0->opt.smoother_precision;
+  m->pre = m0->opt.pre_its;
+  m->post = m0->opt.post_its;
+  m->bc_transfer = m0->opt.bc_transfer;
+  int st = PTMF_OK;
+  auto fail = [&](int code) {
+    ptmf_bmg tmp = m;
+    ptmf_bmg_destroy(&tmp);
+    return code;
+  };
+  if (hipStreamCreate(&m->stream) != hipSuccess) return fail(ptmf_set_error(PTMF_ERR_DEVICE, "bmg_create: stream"));
+  auto up = [&](void **dst, const void *src, size_t bytes) {
+    if (st != PTMF_OK) return;
+    st = ptmf::dalloc(dst, bytes);
+    if (st == PTMF_OK && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+      st = ptmf_set_error(PTMF_ERR_DEVICE, "bmg_create: upload");
+  };
+  for (int l = 0; l < m->L && st == PTMF_OK; l++) {
+    ptmf::BLevel &B = m->lev[l];
+    std::vector<ptmf_op> ops(nbox);
+    std::vector<long> n(nbox), nbc(nbox), dims(6 * nbox, 0);
+    std::vector<const long *> bci(nbox);
+    std::vector<void *> tab((size_t)ptmf::kNRoles * nbox);
+    std::vector<double> scale(nbox), coef((size_t)ptmf::kMaxChebIts * nbox * 3, 0.0);
+    for (int b = 0; b < nbox; b++) {
+      ptmf::MGLevel &Lv = m->mgs[b]->lev[l];
+      ops[b] = Lv.op;
+      n[b] = Lv.n;
+      nbc[b] = Lv.op->nbc;
+      bci[b] = Lv.op->bcidx;
+      void *roles[ptmf::kNRoles] = {Lv.b64, Lv.x64, Lv.r64, Lv.p64[0], Lv.p64[1], Lv.p64[2], Lv.dinv64,
+                                    Lv.b32, Lv.x32, Lv.r32, Lv.p32[0], Lv.p32[1], Lv.p32[2], Lv.dinv32};
+      for (int r = 0; r < ptmf::kNRoles; r++) tab[(size_t)r * nbox + b] = roles[r];
+      if (l > 0) {
+        const ptmf::Box &bf = Lv.op->box, &bc = m->mgs[b]->lev[l - 1].op->box;
+        const long d6[6] = {bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z};
+        for (int k = 0; k < 6; k++) dims[6 * b + k] = d6[k];
+        // the recurrence of the single-box smoother, per box
+        const double emin = Lv.emin, emax = Lv.emax;
+        const double sc = 2.0 / (emax + emin), alpha = 1.0 - sc * emin;
+        const double mu = 1.0 / alpha, omegaprod = 2.0 / alpha;
+        scale[b] = sc;
+        double cm1 = 1.0, c = mu;
+        for (int i = 0; i < ptmf::kMaxChebIts; i++) {
+          const double cp1 = 2.0 * mu * c - cm1;
+          const double omega = omegaprod * c / cp1;
+          double *cf = &coef[((size_t)i * nbox + b) * 3];
+          cf[0] = omega * sc;
+          cf[1] = 1.0 - omega;
+          cf[2] = omega;
+          cm1 = c;
+          c = cp1;
+        }
+      }
+    }
+    if (ptmf_batch_create(ops.data(), nbox, &B.bt) != PTMF_OK) {
+      st = PTMF_ERR_DEVICE;
+      break;
+    }
+    st = ptmf::batch_bind(B.bt, 64);
+    if (st == PTMF_OK) st = ptmf::batch_bind(B.bt, 32);
+    up((void **)&B.dn, n.data(), sizeof(long) * nbox);
+    up((void **)&B.dnbc, nbc.data(), sizeof(long) * nbox);
+    up((void **)&B.bcidx, bci.data(), sizeof(long *) * nbox);
+    up((void **)&B.tab, tab.data(), sizeof(void *) * tab.size());
+    up((void **)&B.dims, dims.data(), sizeof(long) * dims.size());
+    up((void **)&B.scale, scale.data(), sizeof(double) * nbox);
+    up((void **)&B.coef, coef.data(), sizeof(double) * coef.size());
+  }
+  // coarse dense inverses
+  std::vector<const double *> A(nbox);
+  std::vector<long> n0(nbox);
+  for (int b = 0; b < nbox; b++) {
+    A[b] = m->mgs[b]->Adense;
+    n0[b] = m->mgs[b]->ndense;
+    if (st == PTMF_OK && (!A[b] || n0[b] != m->mgs[b]->lev[0].n))
+      st = ptmf_set_error(PTMF_ERR_NOT_SETUP, "bmg_create: multigrid %d has no dense coarse inverse", b);
+  }
+  up((void **)&m->dA, A.data(), sizeof(double *) * nbox);
+  up((void **)&m->dn0, n0.data(), sizeof(long) * nbox);
+  // flexible-CG workspace
+  const ptmf::BLevel &F = m->lev[m->L - 1];
+  (void)F;
+  long tot = 0;
+  std::vector<long> off(nbox);
+  for (int b = 0; b < nbox; b++) {
+    off[b] = tot;
+    tot += m->mgs[b]->lev[m->L - 1].n;
+  }
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&m->fbuf, sizeof(double) * 4 * tot);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&m->ftab, sizeof(void *) * 7 * nbox);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&m->part, sizeof(double) * nbox * ptmf::kSegBlocks * 3);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&m->dscal, sizeof(double) * 2 * nbox);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&m->dactive, sizeof(int) * nbox);
+  if (st == PTMF_OK && hipHostMalloc((void **)&m->hpart, sizeof(double) * nbox * ptmf::kSegBlocks * 3) != hipSuccess)
+    st = ptmf_set_error(PTMF_ERR_NOMEM, "bmg_create: pinned buffer");
+  if (st == PTMF_OK) {
+    // tables 2..6: r, p, q, rold in the workspace, z = the V-cycle output (finest X64)
+    std::vector<void *> ft((size_t)7 * nbox, nullptr);
+    for (int b = 0; b < nbox; b++) {
+      for (int k = 0; k < 4; k++) ft[(size_t)(2 + k) * nbox + b] = m->fbuf + (long)k * tot + off[b];
+      ft[(size_t)6 * nbox + b] = m->mgs[b]->lev[m->L - 1].x64;
+    }
+    if (hipMemcpy(m->ftab, ft.data(), sizeof(void *) * ft.size(), hipMemcpyHostToDevice) != hipSuccess)
+      st = ptmf_set_error(PTMF_ERR_DEVICE, "bmg_create: upload");
+  }
+  if (st != PTMF_OK) return fail(st);
+  *out = m;
+  return PTMF_OK;
+}
+
+// x_b = V-cycle(b_b) for every box (host arrays of device pointers, FP64)
+int ptmf_bmg_vcycle(ptmf_bmg m, const double *const *b, double *const *x) {
+  PTMF_CHECK(m && b && x, PTMF_ERR_ARG, "bmg_vcycle: null");
+  const int nb = m->nbox;
+  const ptmf::BLevel &F = m->lev[m->L - 1];
+  for (int i = 0; i < nb; i++)
+    PTMF_HIP(hipMemcpyAsync(m->mgs[i]->lev[m->L - 1].b64, b[i], sizeof(double) * m->mgs[i]->lev[m->L - 1].n,
+                            hipMemcpyDeviceToDevice, m->stream));
+  PTMF_TRY(ptmf::bvcycle_run(m));
+  for (int i = 0; i < nb; i++)
+    PTMF_HIP(hipMemcpyAsync(x[i], m->mgs[i]->lev[m->L - 1].x64, sizeof(double) * m->mgs[i]->lev[m->L - 1].n,
+                            hipMemcpyDeviceToDevice, m->stream));
+  (void)F;
+  PTMF_HIP(hipStreamSynchronize(m->stream));
+  return PTMF_OK;
+}
+
+// Flexible CG on every box in lockstep, preconditioned by the batched
+// V-cycle, from the given x; box b stops updating once ||r_b|| <= rtol ||b_b||.
+// its[b] and rnorm[b] (true residual at the end) are per box.
+int ptmf_bmg_fcg(ptmf_bmg m, const double *const *b, double *const *x, double rtol, int maxit, int *its,
+                 double *rnorm) {
+  PTMF_CHECK(m && b && x && its && rnorm && maxit > 0, PTMF_ERR_ARG, "bmg_fcg: bad arguments");
+  const int nb = m->nbox, lf = m->L - 1;
+  const ptmf::BLevel &F = m->lev[lf];
+  hipStream_t s = m->stream;
+  // user vectors into tables 0 (b) and 1 (x)
+  std::vector<void *> ux((size_t)2 * nb);
+  for (int i = 0; i < nb; i++) {
+    PTMF_CHECK(b[i] && x[i], PTMF_ERR_ARG, "bmg_fcg: vectors of box %d", i);
+    ux[i] = (void *)b[i];
+    ux[nb + i] = x[i];
+  }
+  PTMF_HIP(hipMemcpyAsync(m->ftab, ux.data(), sizeof(void *) * 2 * nb, hipMemcpyHostToDevice, s));
+  void *const *T = m->ftab;
+  auto t = [&](int k) { return (void *const *)(T + (long)k * nb); };
+  const int tB = 0, tX = 1, tR = 2, tP = 3, tQ = 4, tO = 5, tZ = 6;
+  auto matmult = [&](int src, int dst) {
+    return ptmf::batch_apply(F.bt, 64, (const void *const *)t(src), (void *const *)t(dst), s);
+  };
+  auto to_vcycle = [&]() {  // r -> finest B64
+    ptmf::kb_cast<double, double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(
+        F.dn, (const double *const *)t(tR), ptmf::btab<double>(F, ptmf::kB64, nb));
+    return hipGetLastError() == hipSuccess ? PTMF_OK : ptmf_set_error(PTMF_ERR_DEVICE, "bmg_fcg: launch");
+  };
+  std::vector<double> d;
+  // ||b||
+  PTMF_TRY(ptmf::bdots(m, 1, t(tB), t(tB), nullptr, nullptr, nullptr, nullptr, d));
+  std::vector<double> bn(nb), rz(nb), alpha(nb, 0.0), beta(nb, 0.0);
+  std::vector<int> act(nb, 1);
+  for (int i = 0; i < nb; i++) {
+    bn[i] = sqrt(d[3 * i]);
+    its[i] = 0;
+  }
+  // r = b - A x, z = V(r), p = z
+  PTMF_TRY(matmult(tX, tQ));
+  ptmf::kb_residual<double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(F.dn, (const double *const *)t(tB),
+                                                                       (const double *const *)t(tQ),
+                                                                       (double *const *)t(tR));
+  PTMF_TRY(ptmf::bdots(m, 1, t(tR), t(tR), nullptr, nullptr, nullptr, nullptr, d));
+  int nact = 0;
+  for (int i = 0; i < nb; i++) {
+    rnorm[i] = sqrt(d[3 * i]);
+    act[i] = rnorm[i] > rtol * bn[i];
+    nact += act[i];
+  }
+  if (nact > 0) {
+    PTMF_TRY(to_vcycle());
+    PTMF_TRY(ptmf::bvcycle_run(m));
+    ptmf::kb_cast<double, double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(F.dn, (const double *const *)t(tZ),
+                                                                             (double *const *)t(tP));
+    PTMF_TRY(ptmf::bdots(m, 1, t(tR), t(tZ), nullptr, nullptr, nullptr, nullptr, d));
+    for (int i = 0; i < nb; i++) rz[i] = d[3 * i];
+  }
+  for (int it = 0; it < maxit && nact > 0; it++) {
+    PTMF_TRY(matmult(tP, tQ));
+    PTMF_TRY(ptmf::bdots(m, 1, t(tP), t(tQ), nullptr, nullptr, nullptr, nullptr, d));
+    for (int i = 0; i < nb; i++) {
+      alpha[i] = 0.0;
+      if (!act[i]) continue;
+      const double pq = d[3 * i];
+      PTMF_CHECK(isfinite(pq) && pq != 0.0, PTMF_ERR_DIVERGED, "bmg_fcg: box %d breakdown (p.Ap = %g)", i, pq);
+      alpha[i] = rz[i] / pq;
+    }
+    PTMF_HIP(hipMemcpyAsync(m->dscal, alpha.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    ptmf::kb_fcg_update<<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(
+        F.dn, m->dscal, (const double *const *)t(tP), (const double *const *)t(tQ), (double *const *)t(tX),
+        (double *const *)t(tR), (double *const *)t(tO));
+    PTMF_TRY(to_vcycle());
+    PTMF_TRY(ptmf::bvcycle_run(m));
+    PTMF_TRY(ptmf::bdots(m, 3, t(tR), t(tR), t(tZ), t(tR), t(tZ), t(tO), d));
+    nact = 0;
+    for (int i = 0; i < nb; i++) {
+      if (!act[i]) continue;
+      its[i]++;
+      rnorm[i] = sqrt(d[3 * i]);
+      PTMF_CHECK(isfinite(rnorm[i]), PTMF_ERR_DIVERGED, "bmg_fcg: box %d: non-finite residual", i);
+      if (rnorm[i] <= rtol * bn[i]) {
+        act[i] = 0;
+        continue;
+      }
+      nact++;
+      beta[i] = (d[3 * i + 1] - d[3 * i + 2]) / rz[i];
+      rz[i] = d[3 * i + 1];
+    }
+    PTMF_HIP(hipMemcpyAsync(m->dscal + nb, beta.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    PTMF_HIP(hipMemcpyAsync(m->dactive, act.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+    ptmf::kb_fcg_dir<<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(F.dn, m->dscal + nb, m->dactive,
+                                                                (const double *const *)t(tZ), (double *const *)t(tP));
+    PTMF_HIP(hipGetLastError());
+  }
+  // true residuals
+  PTMF_TRY(matmult(tX, tQ));
+  ptmf::kb_residual<double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(F.dn, (const double *const *)t(tB),
+                                                                       (const double *const *)t(tQ),
+                                                                       (double *const *)t(tR));
+  PTMF_TRY(ptmf::bdots(m, 1, t(tR), t(tR), nullptr, nullptr, nullptr, nullptr, d));
+  for (int i = 0; i < nb; i++) rnorm[i] = sqrt(d[3 * i]);
+  return PTMF_OK;
+}
+
+}  // extern "C"

@@ -244,3 +244,42 @@ def rift_hierarchy(c, nlevels, eta, opt: MGOptions, coarsen="qp-arithmetic", inn
     mg.setup(opt)
     mg.stokes_setup(inner_its, inner_rtol)
     return mg
+
+
+class BatchGMG:
+    """Velocity-block solves of many independent boxes in lockstep: one launch
+    per stage over all boxes, the V-cycle captured once as a HIP graph.  The
+    GMG objects (set up, same levels and smoother options) must outlive it."""
+
+    def __init__(self, mgs):
+        self.mgs = list(mgs)
+        arr = (C.c_void_p * len(self.mgs))(*[g.handle.value for g in self.mgs])
+        h = C.c_void_p()
+        call("ptmf_bmg_create", arr, len(self.mgs), C.byref(h))
+        self.handle = h
+
+    @staticmethod
+    def _ptrs(arrs):
+        return (C.c_void_p * len(arrs))(*[a.ptr for a in arrs])
+
+    def vcycle(self, bs, xs) -> None:
+        call("ptmf_bmg_vcycle", self.handle, self._ptrs(bs), self._ptrs(xs))
+
+    def fcg(self, bs, xs, rtol=1e-8, maxit=500):
+        n = len(self.mgs)
+        its = (C.c_int * n)()
+        rn = (C.c_double * n)()
+        call("ptmf_bmg_fcg", self.handle, self._ptrs(bs), self._ptrs(xs), float(rtol), int(maxit), its, rn)
+        return np.array(its[:]), np.array(rn[:])
+
+    def destroy(self) -> None:
+        if self.handle is not None and self.handle.value:
+            call("ptmf_bmg_destroy", C.byref(self.handle))
+        self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle is not None and self.handle.value and _lib._lib is not None:
+                _lib._lib.ptmf_bmg_destroy(C.byref(self.handle))
+        except Exception:
+            pass
