@@ -73,6 +73,8 @@ def main() -> int:
     ap.add_argument("--solve-m", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--batch", type=int, default=256, help="boxes in the batched-MatMult leg (0: skip)")
+    ap.add_argument("--batch-solve", type=int, default=0,
+                    help="boxes in the batched GMG solve leg (0: skip; the configured workload uses 256)")
     ap.add_argument("--stokes", type=int, nargs=3, metavar=("MX", "MY", "MZ"), default=None,
                     help="run the rift saddle-point leg on this box (e.g. 192 192 96)")
     ap.add_argument("--stokes-maxit", type=int, default=600)
@@ -213,6 +215,8 @@ def main() -> int:
         out["batch"] = batch_leg(args.batch)
     if not args.no_solve:
         out["solve"] = solve_leg(args.solve_m)
+    if args.batch_solve > 0:
+        out["batch_solve"] = batch_solve_leg(args.batch_solve)
     if args.stokes:
         out["stokes"] = stokes_leg(tuple(args.stokes), args.stokes_maxit)
     print(json.dumps(out), flush=True)
@@ -261,6 +265,80 @@ def batch_leg(nbox: int) -> dict:
     B.destroy()
     for A in ops:
         A.destroy()
+    return res
+
+
+def batch_solve_leg(nbox: int) -> dict:
+    """Velocity-block solves of nbox independent boxes (each side drawn from
+    {4, 8, ..., 24} elements, perturbed coordinates, log-uniform viscosity in
+    [1e-3, 1e3], random right-hand side, all-face Dirichlet; seeded): 3-level
+    V-cycle (FP32 Chebyshev(3)-Jacobi, FP64 dense coarse inverse), FP64
+    flexible CG to 1e-8 per box.  Batched: all boxes in lockstep with one
+    launch per stage and the V-cycle replayed as one graph; against each box
+    solved on its own (its own graph-captured V-cycle), one after another."""
+    from ptatin3d_amd import fields as F
+    from ptatin3d_amd._lib import DeviceArray, call
+    from ptatin3d_amd.multigrid import GMG, BatchGMG, default_options
+
+    rng = np.random.default_rng(20261203)
+    mgs, bs, xs = [], [], []
+    E = ndof = 0
+    t0 = time.perf_counter()
+    for _ in range(nbox):
+        m = tuple(int(v) for v in rng.choice([4, 8, 12, 16, 20, 24], size=3))
+        coords = F.perturb_interior(F.box_coords(*m), *m, 0.1, rng)
+        eta = F.eta_loguniform(m[0] * m[1] * m[2], 1e-3, 1e3, rng)
+        mg = GMG(*m, 3)
+        mg.set_coordinates(coords)
+        mg.set_viscosity(2, eta)
+        mg.coarsen_viscosity("qp-arithmetic")
+        mg.set_dirichlet_all_faces()
+        mg.setup(default_options(smoother_precision=32, pre_its=3, post_its=3))
+        mgs.append(mg)
+        b = F.random_velocity(*m, F.dirichlet_all_faces(*m), rng)
+        bs.append(DeviceArray.from_host(b))
+        xs.append(DeviceArray(b.size))
+        E += m[0] * m[1] * m[2]
+        ndof += b.size
+    setup_s = time.perf_counter() - t0
+    bm = BatchGMG(mgs)
+    res = {"boxes": nbox, "elements": E, "velocity_dofs": ndof, "levels": 3,
+           "smoother": "FP32 Chebyshev(3)-Jacobi, FP64 dense coarse inverse, FP64 flexible CG to 1e-8",
+           "setup_seconds_all_boxes": round(setup_s, 2)}
+    for x in xs:
+        x.zero()
+    bm.fcg(bs, xs, rtol=1e-8, maxit=200)  # warm-up: captures the batched V-cycle
+    for x in xs:
+        x.zero()
+    call("ptmf_device_synchronize")
+    t0 = time.perf_counter()
+    its, rn = bm.fcg(bs, xs, rtol=1e-8, maxit=200)
+    call("ptmf_device_synchronize")
+    tb = time.perf_counter() - t0
+    bnorm = np.array([np.linalg.norm(b.download()) for b in bs])
+    res["batched"] = {"seconds": round(tb, 4), "dofs_per_s": round(ndof / tb, 1),
+                      "its_min": int(its.min()), "its_mean": round(float(its.mean()), 2), "its_max": int(its.max()),
+                      "max_rel_residual": float((rn / bnorm).max())}
+    print(f"# batch solve: {nbox} boxes, {tb:.3f} s, its {its.min()}..{its.max()}", file=sys.stderr, flush=True)
+    bm.destroy()
+    # each box on its own (graph-captured V-cycle after the first call)
+    its1 = []
+    for mg, b, x in zip(mgs, bs, xs):
+        x.zero()
+        mg.fcg(b, x, rtol=1e-8, maxit=200)
+    call("ptmf_device_synchronize")
+    t0 = time.perf_counter()
+    for mg, b, x in zip(mgs, bs, xs):
+        x.zero()
+        n, _, _ = mg.fcg(b, x, rtol=1e-8, maxit=200)
+        its1.append(n)
+    call("ptmf_device_synchronize")
+    ts = time.perf_counter() - t0
+    res["per_box"] = {"seconds": round(ts, 4), "dofs_per_s": round(ndof / ts, 1),
+                      "its_mean": round(float(np.mean(its1)), 2)}
+    res["speedup_batched_vs_per_box"] = round(ts / tb, 2)
+    for mg in mgs:
+        mg.destroy()
     return res
 
 

@@ -2052,7 +2052,20 @@ __global__ void kb_prolong_add(const long *__restrict__ dims, const T *const *xc
   }
 }
 
-// y = A x per box with the explicit inverse's lower triangle (column-major)
+// mirror the lower triangle of each box's inverse into its upper triangle
+// (column-major), so the product below reads whole columns
+__global__ void kb_symmetrize(const long *__restrict__ n, double *const *A) {
+  const int b = blockIdx.y;
+  const long nb = n[b];
+  double *Ab = A[b];
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nb * nb; t += (long)gridDim.x * blockDim.x) {
+    const long i = t % nb, j = t / nb;  // entry (i, j), column j
+    if (i < j) Ab[i + j * nb] = Ab[j + i * nb];
+  }
+}
+
+// y = A x per box with the full (symmetrised) inverse: consecutive threads
+// read consecutive rows of a column
 __global__ void kb_symv(const long *__restrict__ n, const double *const *A, const double *const *x, double *const *y) {
   const int b = blockIdx.y;
   const long nb = n[b];
@@ -2060,8 +2073,7 @@ __global__ void kb_symv(const long *__restrict__ n, const double *const *A, cons
   double *yb = y[b];
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nb; i += (long)gridDim.x * blockDim.x) {
     double s = 0.0;
-    for (long j = 0; j <= i; j++) s += Ab[i + j * nb] * xb[j];     // A(i,j), j <= i: column j, row i
-    for (long j = i + 1; j < nb; j++) s += Ab[j + i * nb] * xb[j];  // A(i,j) = A(j,i), j > i
+    for (long j = 0; j < nb; j++) s += Ab[i + j * nb] * xb[j];
     yb[i] = s;
   }
 }
@@ -2418,6 +2430,13 @@ int ptmf_bmg_create(const ptmf_mg *mgs, int nbox, ptmf_bmg *out) {
   }
   up((void **)&m->dA, A.data(), sizeof(double *) * nbox);
   up((void **)&m->dn0, n0.data(), sizeof(long) * nbox);
+  if (st == PTMF_OK) {
+    // the boxes' own dense solves read the lower triangle only; the upper one
+    // is free to hold the mirror image
+    ptmf::kb_symmetrize<<<dim3(256, (unsigned)nbox), ptmf::kBS, 0, m->stream>>>(m->dn0, (double *const *)m->dA);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess)
+      st = ptmf_set_error(PTMF_ERR_DEVICE, "bmg_create: symmetrise");
+  }
   // flexible-CG workspace
   const ptmf::BLevel &F = m->lev[m->L - 1];
   (void)F;
