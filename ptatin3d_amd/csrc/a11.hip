@@ -1082,10 +1082,12 @@ struct BoxDesc {
 template <typename T>
 __global__ void __launch_bounds__(64)
     a11_tile_batch_kernel(const BoxDesc *__restrict__ boxes, const int2 *__restrict__ work, long nwork,
-                          const T *const *__restrict__ xs, T *const *__restrict__ ys) {
+                          const T *const *__restrict__ xs, T *const *__restrict__ ys,
+                          const int *__restrict__ active) {
   __shared__ T sh[3][kSlots][kEPW];
   __shared__ T win[3][3 * kWinNodes];
   const int2 w = work[xcd_tile(blockIdx.x, nwork)];
+  if (active && !active[w.x]) return;  // whole work-group: the box is idle
   const BoxDesc &d = boxes[w.x];
   tile_march<T, true>(d.a, w.y, (const T *)d.xc, (const T *)d.eta, d.bcn, xs[w.x], ys[w.x], (T *)d.fbx,
                       (T *)d.fby, (T *)d.fbz, sh, win);
@@ -1094,8 +1096,9 @@ __global__ void __launch_bounds__(64)
 template <typename T>
 __global__ void __launch_bounds__(64)
     a11_face_batch_kernel(const BoxDesc *__restrict__ boxes, const int2 *__restrict__ work, long nwork,
-                          T *const *__restrict__ ys) {
+                          T *const *__restrict__ ys, const int *__restrict__ active) {
   const int2 w = work[xcd_tile(blockIdx.x, nwork)];
+  if (active && !active[w.x]) return;
   const BoxDesc &d = boxes[w.x];
   face_sum<T>(d.a, w.y, (const T *)d.fbx, (const T *)d.fby, (const T *)d.fbz, ys[w.x]);
 }
@@ -1103,7 +1106,8 @@ __global__ void __launch_bounds__(64)
 // Dirichlet rows y = x of every box (blockIdx.y = box)
 template <typename T>
 __global__ void bc_copy_batch_kernel(const BoxDesc *__restrict__ boxes, const T *const *__restrict__ xs,
-                                     T *const *__restrict__ ys) {
+                                     T *const *__restrict__ ys, const int *__restrict__ active) {
+  if (active && !active[blockIdx.y]) return;
   const BoxDesc &d = boxes[blockIdx.y];
   const T *x = xs[blockIdx.y];
   T *y = ys[blockIdx.y];
@@ -1389,18 +1393,21 @@ int batch_prepare(ptmf_batch_s *b, const void *const *x, void *const *y) {
 
 // the three batched stages on device tables of x and y pointers
 template <typename T>
-int batch_launch_tables(ptmf_batch_s *b, const T *const *xs, T *const *ys, hipStream_t s) {
+int batch_launch_tables(ptmf_batch_s *b, const T *const *xs, T *const *ys, hipStream_t s,
+                        const int *active = nullptr) {
   const int ip = sizeof(T) == 8 ? 0 : 1;
   const int n = (int)b->ops.size();
   ptmf::a11_tile_batch_kernel<T><<<dim3((unsigned)b->nwork), dim3(64), 0, s>>>(b->dbox[ip], b->dwork, b->nwork, xs,
-                                                                              ys);
+                                                                              ys, active);
   PTMF_HIP(hipGetLastError());
-  ptmf::a11_face_batch_kernel<T><<<dim3((unsigned)b->nwork), dim3(64), 0, s>>>(b->dbox[ip], b->dwork, b->nwork, ys);
+  ptmf::a11_face_batch_kernel<T><<<dim3((unsigned)b->nwork), dim3(64), 0, s>>>(b->dbox[ip], b->dwork, b->nwork, ys,
+                                                                              active);
   PTMF_HIP(hipGetLastError());
   if (b->maxnbc > 0) {
     long gx = (b->maxnbc + 255) / 256;
     if (gx > 1024) gx = 1024;
-    ptmf::bc_copy_batch_kernel<T><<<dim3((unsigned)gx, (unsigned)n), dim3(256), 0, s>>>(b->dbox[ip], xs, ys);
+    ptmf::bc_copy_batch_kernel<T><<<dim3((unsigned)gx, (unsigned)n), dim3(256), 0, s>>>(b->dbox[ip], xs, ys,
+                                                                                        active);
     PTMF_HIP(hipGetLastError());
   }
   return PTMF_OK;
@@ -1424,9 +1431,11 @@ int batch_bind(ptmf_batch_s *b, int precision) {
   return PTMF_OK;
 }
 
-int batch_apply(ptmf_batch_s *b, int precision, const void *const *xs, void *const *ys, hipStream_t s) {
-  if (precision == 64) return batch_launch_tables<double>(b, (const double *const *)xs, (double *const *)ys, s);
-  return batch_launch_tables<float>(b, (const float *const *)xs, (float *const *)ys, s);
+int batch_apply(ptmf_batch_s *b, int precision, const void *const *xs, void *const *ys, hipStream_t s,
+                const int *active) {
+  if (precision == 64)
+    return batch_launch_tables<double>(b, (const double *const *)xs, (double *const *)ys, s, active);
+  return batch_launch_tables<float>(b, (const float *const *)xs, (float *const *)ys, s, active);
 }
 }  // namespace ptmf
 

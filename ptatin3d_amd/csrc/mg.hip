@@ -1927,15 +1927,17 @@ constexpr int kSegBlocks = 32;  // blocks per box of the vector stages
 constexpr int kMaxChebIts = 64;
 
 template <typename T>
-__global__ void kb_fill(const long *__restrict__ n, T *const *y, T v) {
+__global__ void kb_fill(const int *__restrict__ act, const long *__restrict__ n, T *const *y, T v) {
   const int b = blockIdx.y;
+  if (act && !act[b]) return;
   T *yb = y[b];
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x) yb[i] = v;
 }
 
 template <typename Ts, typename Td>
-__global__ void kb_cast(const long *__restrict__ n, const Ts *const *x, Td *const *y) {
+__global__ void kb_cast(const int *__restrict__ act, const long *__restrict__ n, const Ts *const *x, Td *const *y) {
   const int b = blockIdx.y;
+  if (act && !act[b]) return;
   const Ts *xb = x[b];
   Td *yb = y[b];
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x)
@@ -1943,8 +1945,9 @@ __global__ void kb_cast(const long *__restrict__ n, const Ts *const *x, Td *cons
 }
 
 template <typename T>
-__global__ void kb_residual(const long *__restrict__ n, const T *const *bb, const T *const *Ax, T *const *r) {
+__global__ void kb_residual(const int *__restrict__ act, const long *__restrict__ n, const T *const *bb, const T *const *Ax, T *const *r) {
   const int b = blockIdx.y;
+  if (act && !act[b]) return;
   const T *bv = bb[b], *av = Ax[b];
   T *rv = r[b];
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n[b]; i += (long)gridDim.x * blockDim.x)
@@ -1953,9 +1956,10 @@ __global__ void kb_residual(const long *__restrict__ n, const T *const *bb, cons
 
 // first Chebyshev step, per-box scale: p = x + scale dinv (b - Ax) (x, Ax may be absent)
 template <typename T>
-__global__ void kb_cheb_first(const long *__restrict__ n, const T *const *bb, const T *const *Ax, const T *const *x,
+__global__ void kb_cheb_first(const int *__restrict__ act, const long *__restrict__ n, const T *const *bb, const T *const *Ax, const T *const *x,
                               const T *const *dinv, const double *__restrict__ scale, T *const *p) {
   const int b = blockIdx.y;
+  if (act && !act[b]) return;
   const T *bv = bb[b], *av = Ax ? Ax[b] : nullptr, *xv = x ? x[b] : nullptr, *dv = dinv[b];
   T *pv = p[b];
   const T sc = (T)scale[b];
@@ -1968,10 +1972,11 @@ __global__ void kb_cheb_first(const long *__restrict__ n, const T *const *bb, co
 
 // p_kp1 = c0 dinv (b - A p_k) + c1 p_km1 + c2 p_k with per-box (c0, c1, c2)
 template <typename T>
-__global__ void kb_cheb_step(const long *__restrict__ n, const T *const *bb, const T *const *Apk, const T *const *dinv,
+__global__ void kb_cheb_step(const int *__restrict__ act, const long *__restrict__ n, const T *const *bb, const T *const *Apk, const T *const *dinv,
                              const T *const *pkm1, const T *const *pk, const double *__restrict__ coef,
                              T *const *pkp1) {
   const int b = blockIdx.y;
+  if (act && !act[b]) return;
   const T *bv = bb[b], *av = Apk[b], *dv = dinv[b], *pm = pkm1[b], *pc = pk[b];
   T *pn = pkp1[b];
   const T c0 = (T)coef[3 * b], c1 = (T)coef[3 * b + 1], c2 = (T)coef[3 * b + 2];
@@ -1980,8 +1985,9 @@ __global__ void kb_cheb_step(const long *__restrict__ n, const T *const *bb, con
 }
 
 template <typename T>
-__global__ void kb_bc_set(const long *__restrict__ nbc, const long *const *idx, T v, T *const *y) {
+__global__ void kb_bc_set(const int *__restrict__ act, const long *__restrict__ nbc, const long *const *idx, T v, T *const *y) {
   const int b = blockIdx.y;
+  if (act && !act[b]) return;
   const long *ib = idx[b];
   T *yb = y[b];
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nbc[b]; i += (long)gridDim.x * blockDim.x)
@@ -1990,8 +1996,9 @@ __global__ void kb_bc_set(const long *__restrict__ nbc, const long *const *idx, 
 
 // per-box lattice sizes: dims[6 b + (0..5)] = Xc, Yc, Zc, Xf, Yf, Zf
 template <typename T>
-__global__ void kb_restrict(const long *__restrict__ dims, const T *const *xf, T *const *yc) {
+__global__ void kb_restrict(const int *__restrict__ act, const long *__restrict__ dims, const T *const *xf, T *const *yc) {
   const int b = blockIdx.y;
+  if (act && !act[b]) return;
   const long *d = dims + 6 * b;
   const long Xc = d[0], Yc = d[1], Zc = d[2], Xf = d[3], Yf = d[4], Zf = d[5];
   const T *xv = xf[b];
@@ -2025,8 +2032,9 @@ __global__ void kb_restrict(const long *__restrict__ dims, const T *const *xf, T
 }
 
 template <typename T>
-__global__ void kb_prolong_add(const long *__restrict__ dims, const T *const *xc, T *const *yf) {
+__global__ void kb_prolong_add(const int *__restrict__ act, const long *__restrict__ dims, const T *const *xc, T *const *yf) {
   const int b = blockIdx.y;
+  if (act && !act[b]) return;
   const long *d = dims + 6 * b;
   const long Xc = d[0], Yc = d[1], Xf = d[3], Yf = d[4], Zf = d[5];
   const T *xv = xc[b];
@@ -2066,8 +2074,9 @@ __global__ void kb_symmetrize(const long *__restrict__ n, double *const *A) {
 
 // y = A x per box with the full (symmetrised) inverse: consecutive threads
 // read consecutive rows of a column
-__global__ void kb_symv(const long *__restrict__ n, const double *const *A, const double *const *x, double *const *y) {
+__global__ void kb_symv(const int *__restrict__ act, const long *__restrict__ n, const double *const *A, const double *const *x, double *const *y) {
   const int b = blockIdx.y;
+  if (act && !act[b]) return;
   const long nb = n[b];
   const double *Ab = A[b], *xb = x[b];
   double *yb = y[b];
@@ -2179,7 +2188,7 @@ T *const *btab(const BLevel &l, int role, int nbox) {
 int bmatmult(ptmf_bmg_s *m, int l, int prec, int src, int dst) {
   const BLevel &B = m->lev[l];
   return batch_apply(B.bt, prec, (const void *const *)(B.tab + (long)src * m->nbox),
-                     (void *const *)(B.tab + (long)dst * m->nbox), m->stream);
+                     (void *const *)(B.tab + (long)dst * m->nbox), m->stream, m->dactive);
 }
 
 template <typename T>
@@ -2192,25 +2201,25 @@ int bcheb(ptmf_bmg_s *m, int l, int rb, int rx, int its, bool nonzero) {
   const int Prole[3] = {rp, rp + 1, rp + 2};
   hipStream_t s = m->stream;
   if (nonzero) {
-    kb_cast<T, T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rx, nb), P[km1]);
+    kb_cast<T, T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dn, btab<T>(B, rx, nb), P[km1]);
     PTMF_TRY(bmatmult(m, l, w, rx, rr));
-    kb_cheb_first<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rb, nb), btab<T>(B, rr, nb), btab<T>(B, rx, nb),
+    kb_cheb_first<T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dn, btab<T>(B, rb, nb), btab<T>(B, rr, nb), btab<T>(B, rx, nb),
                                                   btab<T>(B, rd, nb), B.scale, P[k]);
   } else {
-    kb_fill<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, P[km1], T(0));
-    kb_cheb_first<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rb, nb), nullptr, nullptr, btab<T>(B, rd, nb),
+    kb_fill<T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dn, P[km1], T(0));
+    kb_cheb_first<T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dn, btab<T>(B, rb, nb), nullptr, nullptr, btab<T>(B, rd, nb),
                                                   B.scale, P[k]);
   }
   PTMF_HIP(hipGetLastError());
   for (int i = 0; i < its; i++) {
     PTMF_TRY(bmatmult(m, l, w, Prole[k], rr));
-    kb_cheb_step<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rb, nb), btab<T>(B, rr, nb), btab<T>(B, rd, nb),
+    kb_cheb_step<T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dn, btab<T>(B, rb, nb), btab<T>(B, rr, nb), btab<T>(B, rd, nb),
                                                  P[km1], P[k], B.coef + (long)i * 3 * nb, P[kp1]);
     PTMF_HIP(hipGetLastError());
     const int t = km1;
     km1 = k; k = kp1; kp1 = t;
   }
-  kb_cast<T, T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, P[k], btab<T>(B, rx, nb));
+  kb_cast<T, T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dn, P[k], btab<T>(B, rx, nb));
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }
@@ -2223,11 +2232,11 @@ int bvcycle(ptmf_bmg_s *m, int l, int rb, int rx) {
   const BLevel &B = m->lev[l];
   hipStream_t s = m->stream;
   if (l == 0) {
-    if (sizeof(T) == 4) kb_cast<float, double><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<float>(B, rb, nb),
+    if (sizeof(T) == 4) kb_cast<float, double><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dn, btab<float>(B, rb, nb),
                                                                            btab<double>(B, kB64, nb));
-    kb_symv<<<seg_grid(nb), kBS, 0, s>>>(m->dn0, m->dA, btab<double>(B, sizeof(T) == 4 ? kB64 : rb, nb),
+    kb_symv<<<seg_grid(nb), kBS, 0, s>>>(m->dactive, m->dn0, m->dA, btab<double>(B, sizeof(T) == 4 ? kB64 : rb, nb),
                                          btab<double>(B, sizeof(T) == 4 ? kX64 : rx, nb));
-    if (sizeof(T) == 4) kb_cast<double, float><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<double>(B, kX64, nb),
+    if (sizeof(T) == 4) kb_cast<double, float><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dn, btab<double>(B, kX64, nb),
                                                                            btab<float>(B, rx, nb));
     PTMF_HIP(hipGetLastError());
     return PTMF_OK;
@@ -2236,12 +2245,12 @@ int bvcycle(ptmf_bmg_s *m, int l, int rb, int rx) {
   const int rr = sizeof(T) == 8 ? kR64 : kR32, cb = sizeof(T) == 8 ? kB64 : kB32, cx = sizeof(T) == 8 ? kX64 : kX32;
   PTMF_TRY(bcheb<T>(m, l, rb, rx, m->pre, false));
   PTMF_TRY(bmatmult(m, l, w, rx, rr));
-  kb_residual<T><<<seg_grid(nb), kBS, 0, s>>>(B.dn, btab<T>(B, rb, nb), btab<T>(B, rr, nb), btab<T>(B, rr, nb));
-  kb_restrict<T><<<seg_grid(nb), kBS, 0, s>>>(B.dims, btab<T>(B, rr, nb), btab<T>(C, cb, nb));
-  if (m->bc_transfer) kb_bc_set<T><<<seg_grid(nb), kBS, 0, s>>>(C.dnbc, C.bcidx, T(0), btab<T>(C, cb, nb));
+  kb_residual<T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dn, btab<T>(B, rb, nb), btab<T>(B, rr, nb), btab<T>(B, rr, nb));
+  kb_restrict<T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dims, btab<T>(B, rr, nb), btab<T>(C, cb, nb));
+  if (m->bc_transfer) kb_bc_set<T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, C.dnbc, C.bcidx, T(0), btab<T>(C, cb, nb));
   PTMF_HIP(hipGetLastError());
   PTMF_TRY(bvcycle<T>(m, l - 1, cb, cx));
-  kb_prolong_add<T><<<seg_grid(nb), kBS, 0, s>>>(B.dims, btab<T>(C, cx, nb), btab<T>(B, rx, nb));
+  kb_prolong_add<T><<<seg_grid(nb), kBS, 0, s>>>(m->dactive, B.dims, btab<T>(C, cx, nb), btab<T>(B, rx, nb));
   PTMF_HIP(hipGetLastError());
   return bcheb<T>(m, l, rb, rx, m->post, true);
 }
@@ -2251,9 +2260,9 @@ int bvcycle_body(ptmf_bmg_s *m) {
   const int lf = m->L - 1, nb = m->nbox;
   const BLevel &F = m->lev[lf];
   if (m->prec == 64) return bvcycle<double>(m, lf, kB64, kX64);
-  kb_cast<double, float><<<seg_grid(nb), kBS, 0, m->stream>>>(F.dn, btab<double>(F, kB64, nb), btab<float>(F, kB32, nb));
+  kb_cast<double, float><<<seg_grid(nb), kBS, 0, m->stream>>>(m->dactive, F.dn, btab<double>(F, kB64, nb), btab<float>(F, kB32, nb));
   PTMF_TRY(bvcycle<float>(m, lf, kB32, kX32));
-  kb_cast<float, double><<<seg_grid(nb), kBS, 0, m->stream>>>(F.dn, btab<float>(F, kX32, nb), btab<double>(F, kX64, nb));
+  kb_cast<float, double><<<seg_grid(nb), kBS, 0, m->stream>>>(m->dactive, F.dn, btab<float>(F, kX32, nb), btab<double>(F, kX64, nb));
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }
@@ -2476,7 +2485,12 @@ int ptmf_bmg_vcycle(ptmf_bmg m, const double *const *b, double *const *x) {
   for (int i = 0; i < nb; i++)
     PTMF_HIP(hipMemcpyAsync(m->mgs[i]->lev[m->L - 1].b64, b[i], sizeof(double) * m->mgs[i]->lev[m->L - 1].n,
                             hipMemcpyDeviceToDevice, m->stream));
-  PTMF_TRY(ptmf::bvcycle_run(m));
+  {
+    std::vector<int> all(nb, 1);
+    PTMF_HIP(hipMemcpyAsync(m->dactive, all.data(), sizeof(int) * nb, hipMemcpyHostToDevice, m->stream));
+    PTMF_TRY(ptmf::bvcycle_run(m));
+    PTMF_HIP(hipStreamSynchronize(m->stream));
+  }
   for (int i = 0; i < nb; i++)
     PTMF_HIP(hipMemcpyAsync(x[i], m->mgs[i]->lev[m->L - 1].x64, sizeof(double) * m->mgs[i]->lev[m->L - 1].n,
                             hipMemcpyDeviceToDevice, m->stream));
@@ -2509,7 +2523,7 @@ int ptmf_bmg_fcg(ptmf_bmg m, const double *const *b, double *const *x, double rt
     return ptmf::batch_apply(F.bt, 64, (const void *const *)t(src), (void *const *)t(dst), s);
   };
   auto to_vcycle = [&]() {  // r -> finest B64
-    ptmf::kb_cast<double, double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(
+    ptmf::kb_cast<double, double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(nullptr, 
         F.dn, (const double *const *)t(tR), ptmf::btab<double>(F, ptmf::kB64, nb));
     return hipGetLastError() == hipSuccess ? PTMF_OK : ptmf_set_error(PTMF_ERR_DEVICE, "bmg_fcg: launch");
   };
@@ -2524,7 +2538,7 @@ int ptmf_bmg_fcg(ptmf_bmg m, const double *const *b, double *const *x, double rt
   }
   // r = b - A x, z = V(r), p = z
   PTMF_TRY(matmult(tX, tQ));
-  ptmf::kb_residual<double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(F.dn, (const double *const *)t(tB),
+  ptmf::kb_residual<double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(nullptr, F.dn, (const double *const *)t(tB),
                                                                        (const double *const *)t(tQ),
                                                                        (double *const *)t(tR));
   PTMF_TRY(ptmf::bdots(m, 1, t(tR), t(tR), nullptr, nullptr, nullptr, nullptr, d));
@@ -2534,10 +2548,12 @@ int ptmf_bmg_fcg(ptmf_bmg m, const double *const *b, double *const *x, double rt
     act[i] = rnorm[i] > rtol * bn[i];
     nact += act[i];
   }
+  // converged boxes sit out the V-cycle (its stage kernels skip them)
+  PTMF_HIP(hipMemcpyAsync(m->dactive, act.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
   if (nact > 0) {
     PTMF_TRY(to_vcycle());
     PTMF_TRY(ptmf::bvcycle_run(m));
-    ptmf::kb_cast<double, double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(F.dn, (const double *const *)t(tZ),
+    ptmf::kb_cast<double, double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(nullptr, F.dn, (const double *const *)t(tZ),
                                                                              (double *const *)t(tP));
     PTMF_TRY(ptmf::bdots(m, 1, t(tR), t(tZ), nullptr, nullptr, nullptr, nullptr, d));
     for (int i = 0; i < nb; i++) rz[i] = d[3 * i];
@@ -2581,7 +2597,7 @@ int ptmf_bmg_fcg(ptmf_bmg m, const double *const *b, double *const *x, double rt
   }
   // true residuals
   PTMF_TRY(matmult(tX, tQ));
-  ptmf::kb_residual<double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(F.dn, (const double *const *)t(tB),
+  ptmf::kb_residual<double><<<ptmf::seg_grid(nb), ptmf::kBS, 0, s>>>(nullptr, F.dn, (const double *const *)t(tB),
                                                                        (const double *const *)t(tQ),
                                                                        (double *const *)t(tR));
   PTMF_TRY(ptmf::bdots(m, 1, t(tR), t(tR), nullptr, nullptr, nullptr, nullptr, d));

@@ -148,9 +148,11 @@ int launch_eta_qpmajor(long nel, const double *em, double *q64, float *q32, hipS
 int launch_stokes_mult(ptmf_op_s *op, const double *x, double *y);
 // batched MatMult on device tables of per-box x / y pointers (precision 32 or
 // 64): batch_bind builds the work list and box descriptors once (and the
-// tile-ordered viscosity), batch_apply launches on stream s (capturable)
+// tile-ordered viscosity), batch_apply launches on stream s (capturable);
+// boxes whose active[box] is 0 are skipped (active may be null: all boxes)
 int batch_bind(ptmf_batch_s *b, int precision);
-int batch_apply(ptmf_batch_s *b, int precision, const void *const *xs, void *const *ys, hipStream_t s);
+int batch_apply(ptmf_batch_s *b, int precision, const void *const *xs, void *const *ys, hipStream_t s,
+                const int *active = nullptr);
 int launch_powerlaw(const ptmf_op_s *op, const double *u, const double *B, double nexp, double eta_min,
                     double eta_max, double *eta);
 int launch_a11_diag(const ptmf_op_s *op, double *d);
