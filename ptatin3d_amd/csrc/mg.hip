@@ -669,6 +669,23 @@ __global__ void k_probe_scatter(long X, long Y, long Z, int ci, int cj, int ck, 
   }
 }
 
+// the same probe scattered into a dense column-major matrix of order 3N:
+// entry (3m + r, 3n + d) = w[3m + r], n the colour's node within +-2 of m
+__global__ void k_probe_scatter_dense(long X, long Y, long Z, int ci, int cj, int ck, int d,
+                                      const double *__restrict__ w, double *__restrict__ A) {
+  const long N = X * Y * Z, n3 = 3 * N;
+  for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < N; m += (long)gridDim.x * blockDim.x) {
+    const long i = m % X, j = (m / X) % Y, k = m / (X * Y);
+    int oi = (int)(((ci - i) % 5 + 5) % 5), oj = (int)(((cj - j) % 5 + 5) % 5), ok = (int)(((ck - k) % 5 + 5) % 5);
+    if (oi > 2) oi -= 5;
+    if (oj > 2) oj -= 5;
+    if (ok > 2) ok -= 5;
+    if (i + oi < 0 || i + oi >= X || j + oj < 0 || j + oj >= Y || k + ok < 0 || k + ok >= Z) continue;
+    const long col = 3 * (m + oi + X * (oj + Y * ok)) + d;
+    for (int r = 0; r < 3; r++) A[(3 * m + r) + col * n3] = w[3 * m + r];
+  }
+}
+
 // Dirichlet dof g = 3m + d of a Galerkin level: identity row (its row and
 // column entries are already zero from the masked probes)
 __global__ void k_stencil_bc_identity(long N, const long *__restrict__ idx, long nbc, double *__restrict__ S) {
@@ -1158,15 +1175,22 @@ int assemble_dense(ptmf_mg_s *mg) {
   mg->ndense = (int)n;
   if (!mg->Adense) PTMF_TRY(dalloc((void **)&mg->Adense, sizeof(double) * n * n));
   hipStream_t s = mg->stream;
-  std::vector<double> e(n, 0.0);
-  double *ej = L.p64[0];
-  for (long j = 0; j < n; j++) {
-    e[j] = 1.0;
-    PTMF_HIP(hipMemcpyAsync(ej, e.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-    PTMF_TRY(matmult<double>(L, ej, mg->Adense + j * n));
-    PTMF_HIP(hipStreamSynchronize(s));
-    e[j] = 0.0;
-  }
+  // 375 coloured probes instead of n unit vectors: nodes of one colour
+  // (i, j, k mod 5 fixed) are 5 apart, so their columns of the level
+  // operator (couplings within +-2 nodes) do not overlap
+  const Box &b0 = L.op->box;
+  double *v = L.p64[0], *w = L.r64;
+  PTMF_HIP(hipMemsetAsync(mg->Adense, 0, sizeof(double) * n * n, s));
+  for (int ck = 0; ck < 5; ck++)
+    for (int cj = 0; cj < 5; cj++)
+      for (int ci = 0; ci < 5; ci++)
+        for (int d = 0; d < 3; d++) {
+          k_probe<<<nblocks(b0.nnodes), kBS, 0, s>>>(b0.X, b0.Y, b0.Z, ci, cj, ck, d, v);
+          PTMF_TRY(matmult<double>(L, v, w));
+          k_probe_scatter_dense<<<nblocks(b0.nnodes), kBS, 0, s>>>(b0.X, b0.Y, b0.Z, ci, cj, ck, d, w, mg->Adense);
+          PTMF_HIP(hipGetLastError());
+        }
+  PTMF_HIP(hipStreamSynchronize(s));
   if (!mg->dinfo) PTMF_TRY(dalloc((void **)&mg->dinfo, sizeof(int)));
   rocblas_status st = rocsolver_dpotrf(mg->blas, rocblas_fill_lower, (int)n, mg->Adense, (int)n, mg->dinfo);
   PTMF_CHECK(st == rocblas_status_success, PTMF_ERR_DEVICE, "rocsolver_dpotrf: status %d", (int)st);
