@@ -782,6 +782,7 @@ struct ptmf_mg_s {
   double *pcg[4] = {nullptr, nullptr, nullptr, nullptr};
   // saddle point: inverse Schur blocks (16 per element) and a velocity temporary
   double *sMinv = nullptr, *stmp = nullptr;
+  double *sx2 = nullptr, *sy2 = nullptr;  // [u | p] temporaries of the preconditioner's A12 product
   bool stokes_ready = false;
   // captured V-cycle (levels_mg == L): runs between the fixed buffers gb, gx
   hipGraph_t vgraph = nullptr;
@@ -1339,11 +1340,15 @@ int stokes_apply(ptmf_mg_s *mg, const double *x, double *y) {
 int stokes_prec(ptmf_mg_s *mg, const double *r, double *z) {
   MGLevel &F = mg->lev[mg->L - 1];
   const long nu = F.n;
+  const long np = 4 * F.op->box.nel;
   PTMF_TRY(launch_schur_apply(F.op, mg->sMinv, r + nu, z + nu));
-  PTMF_TRY(launch_fill<double>(mg->stmp, 0.0, nu, mg->stream));
-  PTMF_TRY(launch_a12(F.op, z + nu, mg->stmp));
-  PTMF_TRY(launch_bc_set<double>(F.op->bcidx, F.op->nbc, 0.0, mg->stmp, mg->stream));
-  PTMF_TRY(axpby<double>(mg->stream, nu, 1.0, r, -1.0, mg->stmp));
+  // stmp = r_u - A12 z_p, A12 z_p from the fused saddle-point kernel on
+  // [0 | z_p] (one element pass; its Dirichlet rows come out as 0)
+  PTMF_TRY(launch_fill<double>(mg->sx2, 0.0, nu, mg->stream));
+  PTMF_HIP(hipMemcpyAsync(mg->sx2 + nu, z + nu, sizeof(double) * np, hipMemcpyDeviceToDevice, mg->stream));
+  PTMF_TRY(launch_stokes_mult(F.op, mg->sx2, mg->sy2));
+  k_residual<double><<<nblocks(nu), kBS, 0, mg->stream>>>(nu, r, mg->sy2, mg->stmp);
+  PTMF_HIP(hipGetLastError());
   if (mg->s_inner_its <= 1) return ptmf_mg_vcycle(mg, mg->stmp, z);
   PTMF_HIP(hipMemsetAsync(z, 0, sizeof(double) * nu, mg->stream));
   int its = 0;
@@ -1458,6 +1463,8 @@ int ptmf_mg_destroy(ptmf_mg *pmg) {
   mg->ks.release();
   (void)hipFree(mg->sMinv);
   (void)hipFree(mg->stmp);
+  (void)hipFree(mg->sx2);
+  (void)hipFree(mg->sy2);
   ptmf::drop_graph(mg);
   (void)hipFree(mg->gb);
   (void)hipFree(mg->gx);
@@ -1865,6 +1872,9 @@ int ptmf_mg_stokes_setup(ptmf_mg mg, int inner_its, double inner_rtol) {
   MGLevel &F = mg->lev[mg->L - 1];
   if (!mg->sMinv) PTMF_TRY(ptmf::dalloc((void **)&mg->sMinv, sizeof(double) * 16 * F.op->box.nel));
   if (!mg->stmp) PTMF_TRY(ptmf::dalloc((void **)&mg->stmp, sizeof(double) * F.n));
+  const long nup = F.n + 4 * F.op->box.nel;
+  if (!mg->sx2) PTMF_TRY(ptmf::dalloc((void **)&mg->sx2, sizeof(double) * nup));
+  if (!mg->sy2) PTMF_TRY(ptmf::dalloc((void **)&mg->sy2, sizeof(double) * nup));
   PTMF_TRY(ptmf::launch_schur_blocks(F.op, nullptr, mg->sMinv));
   PTMF_HIP(hipStreamSynchronize(mg->stream));
   mg->stokes_ready = true;
