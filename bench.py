@@ -135,6 +135,15 @@ def main() -> int:
     for _ in range(5):
         A.get_diagonal(d)
     ms_diag = 1e3 * (time.perf_counter() - t2) / 5
+    # the reference's host-array contract (Yu += A u on host arrays, copies
+    # over PCIe included): reported beside, never as the value
+    uh = np.ascontiguousarray(c["u"])
+    Yu = np.zeros(ndof)
+    A.spmv_host(uh, Yu)
+    t3 = time.perf_counter()
+    for _ in range(5):
+        A.spmv_host(uh, Yu)
+    ms_host = 1e3 * (time.perf_counter() - t3) / 5
 
     gdofs = ndof / (ms_step * 1e-3) / 1e9
     out = {
@@ -177,6 +186,8 @@ def main() -> int:
         },
         "timing": {"ms_event": round(ms_event, 5), "ms_wall": round(ms_wall, 5),
                    "ms_diagonal": round(ms_diag, 3),
+                   "ms_host_array_spmv_pcie": round(ms_host, 3),
+                   "host_array_GDoF_s": round(ndof / (ms_host * 1e-3) / 1e9, 3),
                    "diagonal_GBs": round(8 * (6 * F.nnodes(m, m, m) + 27 * m ** 3) / (ms_diag * 1e-3) / 1e9, 1)},
     }
 
