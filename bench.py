@@ -330,6 +330,7 @@ def batch_solve_leg(nbox: int) -> dict:
     call("ptmf_device_synchronize")
     tb = time.perf_counter() - t0
     bnorm = np.array([np.linalg.norm(b.download()) for b in bs])
+    xb = [x.download() for x in xs]
     res["batched"] = {"seconds": round(tb, 4), "dofs_per_s": round(ndof / tb, 1),
                       "its_min": int(its.min()), "its_mean": round(float(its.mean()), 2), "its_max": int(its.max()),
                       "max_rel_residual": float((rn / bnorm).max())}
@@ -348,8 +349,10 @@ def batch_solve_leg(nbox: int) -> dict:
         its1.append(n)
     call("ptmf_device_synchronize")
     ts = time.perf_counter() - t0
+    diff = max(float(np.linalg.norm(x.download() - y) / max(np.linalg.norm(y), 1e-300)) for x, y in zip(xs, xb))
     res["per_box"] = {"seconds": round(ts, 4), "dofs_per_s": round(ndof / ts, 1),
-                      "its_mean": round(float(np.mean(its1)), 2)}
+                      "its_mean": round(float(np.mean(its1)), 2),
+                      "max_rel_diff_vs_batched": diff}
     res["speedup_batched_vs_per_box"] = round(ts / tb, 2)
     for mg in mgs:
         mg.destroy()
