@@ -225,6 +225,7 @@ def main() -> int:
         dst.free()
 
     A.destroy()
+    out["kernels"] = kernels_leg(m)
     if args.batch > 0:
         out["batch"] = batch_leg(args.batch)
     if not args.no_solve:
@@ -279,6 +280,58 @@ def batch_leg(nbox: int) -> dict:
     B.destroy()
     for A in ops:
         A.destroy()
+    return res
+
+
+def kernels_leg(m: int) -> dict:
+    """The multigrid's other kernels on the m^3 workload (finest two levels of a
+    hierarchy down to 8^3): the
+    transfers against their algorithmic bytes, prolong-add s (3 N_c + 6 N_f)
+    and restrict s (3 N_f + 3 N_c), and one Chebyshev-Jacobi step (one A11
+    MatMult plus the vector streams) in FP64 and FP32; wall clock over
+    repeated synchronised calls."""
+    from ptatin3d_amd import fields as F
+    from ptatin3d_amd._lib import DeviceArray, call
+    from ptatin3d_amd.multigrid import GMG, default_options
+
+    c = F.config2(m)
+    nl = 1
+    while (m >> nl) >= 8 and nl < 4:
+        nl += 1
+    lf = nl - 1
+    mg = GMG(m, m, m, nl)
+    mg.set_coordinates(c["coords"])
+    mg.set_viscosity(lf, c["eta"])
+    mg.coarsen_viscosity("qp-arithmetic")
+    mg.set_dirichlet_all_faces()
+    mg.setup(default_options(smoother_precision=64))
+    nf, nc = F.nnodes(m, m, m), F.nnodes(m // 2, m // 2, m // 2)
+    xf, xc = DeviceArray(3 * nf), DeviceArray(3 * nc)
+    xf.upload(np.random.default_rng(1).uniform(-1, 1, 3 * nf))
+    xc.upload(np.random.default_rng(2).uniform(-1, 1, 3 * nc))
+
+    def timed(f, reps=20):
+        f()
+        call("ptmf_device_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        call("ptmf_device_synchronize")
+        return 1e3 * (time.perf_counter() - t0) / reps
+
+    res = {"workload": f"{m}^3 -> {m // 2}^3, FP64 transfers"}
+    ms = timed(lambda: mg.prolong_add(lf, xc, xf))
+    res["prolong_add"] = {"ms": round(ms, 4), "GBs": round(8 * (3 * nc + 6 * nf) / (ms * 1e-3) / 1e9, 1)}
+    ms = timed(lambda: mg.restrict(lf, xf, xc))
+    res["restrict"] = {"ms": round(ms, 4), "GBs": round(8 * (3 * nf + 3 * nc) / (ms * 1e-3) / 1e9, 1)}
+    for prec, dt in ((64, np.float64), (32, np.float32)):
+        b = DeviceArray.from_host(c["u"].astype(dt))
+        x = DeviceArray(b.n, dt)
+        x.zero()
+        ms1 = timed(lambda: mg.smooth(lf, b, x, 1, True), 10)
+        ms5 = timed(lambda: mg.smooth(lf, b, x, 6, True), 10)
+        res[f"chebyshev_step_fp{prec}_ms"] = round((ms5 - ms1) / 5, 4)
+    mg.destroy()
     return res
 
 
