@@ -78,6 +78,8 @@ def main() -> int:
     ap.add_argument("--stokes", type=int, nargs=3, metavar=("MX", "MY", "MZ"), default=None,
                     help="run the rift saddle-point leg on this box (e.g. 192 192 96)")
     ap.add_argument("--stokes-maxit", type=int, default=600)
+    ap.add_argument("--stokes-schur", choices=("mass", "bdb"), default="mass",
+                    help="Schur approximation of the rift leg: scaled viscous mass or B^T diag(A11)^-1 B blocks")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -233,7 +235,7 @@ def main() -> int:
     if args.batch_solve > 0:
         out["batch_solve"] = batch_solve_leg(args.batch_solve)
     if args.stokes:
-        out["stokes"] = stokes_leg(tuple(args.stokes), args.stokes_maxit)
+        out["stokes"] = stokes_leg(tuple(args.stokes), args.stokes_maxit, args.stokes_schur)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -459,15 +461,15 @@ def solve_leg(m: int) -> dict:
     return res
 
 
-def stokes_leg(dims, maxit: int) -> dict:
+def stokes_leg(dims, maxit: int, schur: str = "mass") -> dict:
     """Rift saddle point: a 1e3 crust (z > 0.75) over a 1e0 mantle with a
     seeded 1e-3 notch, power-law n = 3 viscosity of a seeded extension
     velocity at the quadrature points (clipped to [1e-3, 1e3]), gravity load;
     free-slip sides and base, extension velocity on the x-faces, free top.
     FGMRES(40) in FP64 on [u | p] with the fused saddle-point MatMult and the
-    upper block-triangular preconditioner (scaled viscous-mass Schur
-    approximation, one 5-level V-cycle for A11: Chebyshev(4)-Jacobi, FP32 or
-    FP64).  Reports the fused MatMult rate, then both solves to 1e-8 or maxit."""
+    upper block-triangular preconditioner (Schur approximation: the scaled
+    viscous mass, or element blocks of B^T diag(A11)^-1 B; one 5-level
+    V-cycle for A11: Chebyshev(4)-Jacobi, FP32 or FP64).  Reports the fused MatMult rate, then both solves to 1e-8 or maxit."""
     from ptatin3d_amd import fields as F
     from ptatin3d_amd._lib import DeviceArray, call
     from ptatin3d_amd.multigrid import (default_options, powerlaw_viscosity, rift_hierarchy,
@@ -487,7 +489,9 @@ def stokes_leg(dims, maxit: int) -> dict:
                        "free-slip sides and base, x-face extension, free top",
            "velocity_dofs": nu, "pressure_dofs": 4 * E, "levels": nl,
            "krylov": "FGMRES(40), FP64, fused saddle-point MatMult",
-           "preconditioner": "upper block-triangular: scaled viscous-mass Schur, one V-cycle for A11"}
+           "preconditioner": "upper block-triangular: " + {"mass": "scaled viscous-mass",
+                                                            "bdb": "B^T diag(A11)^-1 B element-block"}[schur]
+                             + " Schur, one V-cycle for A11"}
     # fused saddle-point MatMult throughput
     A = MatStokesMF(mx, my, mz).setup(c["coords"], eta, c["bc"])
     x = DeviceArray.from_host(np.concatenate([bu, bp]))
@@ -511,7 +515,7 @@ def stokes_leg(dims, maxit: int) -> dict:
     for tag, prec in (("fp32", 32), ("fp64", 64)):
         t0 = time.perf_counter()
         mg = rift_hierarchy(c, nl, eta, default_options(smoother_precision=prec, pre_its=4, post_its=4,
-                                                          esteig=(0.0, 0.1, 0.0, 1.1)))
+                                                          esteig=(0.0, 0.1, 0.0, 1.1)), schur=schur)
         ts = time.perf_counter() - t0
         u, p, its, rn, hist, secs = stokes_solve_timed(mg, bu, bp, rtol=1e-8, maxit=maxit, restart=40)
         mg.destroy()

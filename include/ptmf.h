@@ -303,6 +303,10 @@ int ptmf_bmg_fcg(ptmf_bmg bmg, const double *const *b, double *const *x, double 
  * reference's inner -fieldsplit_u_ksp_type fgmres -fieldsplit_u_ksp_rtol.
  * ptmf_mg_stokes_setup needs ptmf_mg_setup first. */
 int ptmf_mg_stokes_setup(ptmf_mg mg, int inner_its, double inner_rtol);
+/* Schur approximation: 0 scaled viscous mass (the reference's, default);
+ * 1 element blocks of B^T D^-1 B, D the A11 diagonal (no counterpart in the
+ * reference). */
+int ptmf_mg_stokes_set_schur(ptmf_mg mg, int type);
 int ptmf_mg_stokes_apply(ptmf_mg mg, const double *x, double *y);
 int ptmf_mg_stokes_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int restart,
                           int *its, double *rnorm, double *history, int history_len);

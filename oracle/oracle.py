@@ -111,6 +111,37 @@ def schur_mass_blocks(mx, my, mz, coords, eta):
     return M.reshape(-1, 4, 4)
 
 
+def schur_bdb_blocks(mx, my, mz, coords, eta, bc):
+    """Inverse element blocks (-B_e^T D^-1 B_e)^-1 of the Schur approximation
+    built from the pressure-gradient columns B = A12 (orc_a12_matmult) and
+    the A11 diagonal D (orc_a11_diagonal), Dirichlet rows left out.  The P1
+    pressure is discontinuous, so column 4e+i of the assembled A12 holds
+    element e's contribution only; the columns of all elements of one parity
+    class (ei, ej, ek mod 2) touch disjoint nodes and come from one product."""
+    X, Y = 2 * mx + 1, 2 * my + 1
+    E = mx * my * mz
+    d = a11_diagonal(mx, my, mz, coords, eta, bc)
+    free = ~np.asarray(bc).astype(bool)
+    l = np.arange(3)
+    loc = (l[None, None, :] + X * l[None, :, None] + X * Y * l[:, None, None]).reshape(-1)
+    S = np.zeros((E, 4, 4))
+    cols = np.zeros((E, 81, 4))
+    e = np.arange(E)
+    ei, ej, ek = e % mx, (e // mx) % my, e // (mx * my)
+    nodes = (2 * ei + X * 2 * ej + X * Y * 2 * ek)[:, None] + loc[None, :]
+    dofs = (3 * nodes[:, :, None] + np.arange(3)[None, None, :]).reshape(E, 81)
+    for cls in range(8):
+        sel = (ei % 2 == cls & 1) & (ej % 2 == (cls >> 1) & 1) & (ek % 2 == (cls >> 2) & 1)
+        for i in range(4):
+            p = np.zeros(4 * E)
+            p[4 * e[sel] + i] = 1.0
+            y = a12_matmult(mx, my, mz, coords, bc, p)
+            cols[sel, :, i] = y[dofs[sel]]
+    w = np.where(free[dofs], 1.0 / np.where(free, d, 1.0)[dofs], 0.0)
+    S = np.einsum("eri,er,erj->eij", cols, w, cols)
+    return np.linalg.inv(-S)
+
+
 def prolong_add(mxc, myc, mzc, xc, yf):
     load().orc_prolong_add(mxc, myc, mzc, _p(_f64(xc)), _p(yf))
     return yf

@@ -94,6 +94,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_bmg_vcycle": (_I, [_P, _P, _P]),
     "ptmf_bmg_fcg": (_I, [_P, _P, _P, _D, _I, _P, _P]),
     "ptmf_mg_stokes_setup": (_I, [_P, _I, _D]),
+    "ptmf_mg_stokes_set_schur": (_I, [_P, _I]),
     "ptmf_mg_stokes_apply": (_I, [_P, _P, _P]),
     "ptmf_mg_stokes_fgmres": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _PD, _PD, _I]),
     "ptmf_schur_mass_blocks": (_I, [_P, _P]),

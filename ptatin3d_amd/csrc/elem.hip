@@ -466,6 +466,105 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// ---- element blocks of B^T D^-1 B with D the assembled A11 diagonal:
+// S_e[i][j] = sum_{n,d not Dirichlet} B_e[(n,d),i] B_e[(n,d),j] / D[3 node + d],
+// B_e[(n,d),i] = -sum_q w detJ (d_d N_n) NIp_i (the element's A12 columns);
+// Minv_e = (-S_e)^-1.  An element-local stand-in for the Schur complement
+// -B^T A^-1 B that sees the stiffness of the neighbours through D.
+__global__ void __launch_bounds__(64)
+    schur_bdb_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
+                     const uint8_t *__restrict__ bcn, const double *__restrict__ dg, double *__restrict__ Minv) {
+  __shared__ double sx[kEPB][27][3];
+  __shared__ double sG[kEPB][27][14];  // G (9), -w detJ, NIp (4)
+  __shared__ double sr[kEPB][32][10];
+  const int h = threadIdx.x >> 5, t = threadIdx.x & 31;
+  long e = (long)blockIdx.x * kEPB + h;
+  const bool live = e < nel;
+  if (!live) e = nel - 1;
+  if (t < 27) {
+    const long node = elem_node(mx, my, X, XY, e, t);
+    for (int d = 0; d < 3; d++) sx[h][t][d] = xc[3 * node + d];
+  }
+  __syncthreads();
+  if (t < 27) {
+    QpGeom g;
+    qp_geometry(sx[h], t, g);
+    double Np[4];
+    p1_basis(sx[h], g.xq, Np);
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) sG[h][t][3 * r + c] = g.G[r][c];
+    sG[h][t][9] = -g.wdet;
+    for (int i = 0; i < 4; i++) sG[h][t][10 + i] = Np[i];
+  }
+  __syncthreads();
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (t < 27) {
+    LaneTab T;
+    lane_tab_node(t, T);
+    double Bn[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll 1
+    for (int qc = 0; qc < 3; qc++) {
+      const double bk = pick3(qc, T.B[2][0], T.B[2][1], T.B[2][2]), dk = pick3(qc, T.D[2][0], T.D[2][1], T.D[2][2]);
+#pragma unroll
+      for (int qb = 0; qb < 3; qb++)
+#pragma unroll
+        for (int qa = 0; qa < 3; qa++) {
+          const double *G = sG[h][qa + 3 * qb + 9 * qc];
+          const double gr[3] = {T.D[0][qa] * T.B[1][qb] * bk, T.B[0][qa] * T.D[1][qb] * bk,
+                                T.B[0][qa] * T.B[1][qb] * dk};
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            const double gd = G[9] * (G[3 * d] * gr[0] + G[3 * d + 1] * gr[1] + G[3 * d + 2] * gr[2]);
+#pragma unroll
+            for (int i = 0; i < 4; i++) Bn[d][i] += gd * G[10 + i];
+          }
+        }
+    }
+    const long node = elem_node(mx, my, X, XY, e, t);
+    const unsigned m = bcn ? (unsigned)bcn[node] : 0u;
+    for (int d = 0; d < 3; d++) {
+      if ((m >> d) & 1u) continue;
+      const double id = 1.0 / dg[3 * node + d];
+      int k = 0;
+      for (int i = 0; i < 4; i++)
+        for (int j = i; j < 4; j++) v[k++] += Bn[d][i] * Bn[d][j] * id;
+    }
+  }
+  for (int k = 0; k < 10; k++) sr[h][t][k] = v[k];
+  __syncthreads();
+  if (t == 0 && live) {
+    double A[4][4];
+    int k = 0;
+    for (int i = 0; i < 4; i++)
+      for (int j = i; j < 4; j++) {
+        double s = 0.0;
+        for (int q = 0; q < 27; q++) s += sr[h][q][k];
+        A[i][j] = A[j][i] = -s;
+        k++;
+      }
+    double B[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    for (int c = 0; c < 4; c++) {
+      int pr = c;
+      for (int r = c + 1; r < 4; r++)
+        if (fabs(A[r][c]) > fabs(A[pr][c])) pr = r;
+      for (int j = 0; j < 4; j++) {
+        const double ta = A[c][j], tb = B[c][j];
+        A[c][j] = A[pr][j]; B[c][j] = B[pr][j];
+        A[pr][j] = ta; B[pr][j] = tb;
+      }
+      const double ip = 1.0 / A[c][c];
+      for (int j = 0; j < 4; j++) { A[c][j] *= ip; B[c][j] *= ip; }
+      for (int r = 0; r < 4; r++) {
+        if (r == c) continue;
+        const double f = A[r][c];
+        for (int j = 0; j < 4; j++) { A[r][j] -= f * A[c][j]; B[r][j] -= f * B[c][j]; }
+      }
+    }
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) Minv[16 * e + 4 * i + j] = B[i][j];
+  }
+}
+
 // yp[4e+i] = sum_j Minv_e[i][j] rp[4e+j] (+ beta yp when beta != 0)
 __global__ void schur_apply_kernel(long nel, const double *__restrict__ Minv, const double *__restrict__ rp,
                                    double *__restrict__ yp) {
@@ -571,6 +670,16 @@ int launch_schur_blocks(const ptmf_op_s *op, double *Me, double *Minv) {
   const long nblk = (b.nel + kEPB - 1) / kEPB;
   schur_blocks_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(
       b.mx, b.my, b.nel, b.X, b.X * b.Y, op->d64.coords, op->d64.eta, Me, Minv);
+  PTMF_HIP(hipGetLastError());
+  return PTMF_OK;
+}
+
+int launch_schur_bdb(const ptmf_op_s *op, const double *dg, double *Minv) {
+  PTMF_TRY(init_constants());
+  const Box &b = op->box;
+  const long nblk = (b.nel + kEPB - 1) / kEPB;
+  schur_bdb_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(b.mx, b.my, b.nel, b.X, b.X * b.Y,
+                                                                      op->d64.coords, op->bcnode, dg, Minv);
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }

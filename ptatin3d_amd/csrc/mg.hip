@@ -1897,6 +1897,25 @@ int ptmf_mg_stokes_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, i
 }
 
 // y = K x for the saddle point ([u | p] device vectors)
+// Schur approximation of the saddle-point preconditioner: 0 the scaled
+// viscous mass (default, set by ptmf_mg_stokes_setup), 1 element blocks of
+// B^T D^-1 B with D the finest level's A11 diagonal
+int ptmf_mg_stokes_set_schur(ptmf_mg mg, int type) {
+  PTMF_CHECK(mg && mg->sMinv && mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_stokes_set_schur: needs ptmf_mg_stokes_setup");
+  PTMF_CHECK(type == 0 || type == 1, PTMF_ERR_UNSUPPORTED, "mg_stokes_set_schur: type %d", type);
+  ptmf::MGLevel &F = mg->lev[mg->L - 1];
+  if (type == 0) PTMF_TRY(ptmf::launch_schur_blocks(F.op, nullptr, mg->sMinv));
+  else {
+    // the plain diagonal (the smoother's may be the l1 row sums), cleared
+    // before the element sums; stmp is free outside a preconditioner
+    // application
+    PTMF_TRY(ptmf_a11_diagonal(F.op, mg->stmp));
+    PTMF_TRY(ptmf::launch_schur_bdb(F.op, mg->stmp, mg->sMinv));
+  }
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  return PTMF_OK;
+}
+
 int ptmf_mg_stokes_apply(ptmf_mg mg, const double *x, double *y) {
   PTMF_CHECK(mg && x && y, PTMF_ERR_ARG, "mg_stokes_apply: null");
   PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_stokes_apply: not set up");

@@ -134,6 +134,12 @@ class GMG:
         or up to inner_its V-cycle-preconditioned flexible-CG iterations."""
         call("ptmf_mg_stokes_setup", self.handle, int(inner_its), float(inner_rtol))
 
+    def set_schur(self, kind: str) -> None:
+        """Schur approximation of the saddle-point preconditioner: "mass"
+        (inverse-viscosity-scaled pressure mass, the default) or "bdb"
+        (element blocks of B^T diag(A11)^-1 B).  Call after stokes_setup."""
+        call("ptmf_mg_stokes_set_schur", self.handle, {"mass": 0, "bdb": 1}[kind])
+
     def stokes_apply(self, x: DeviceArray, y: DeviceArray) -> None:
         call("ptmf_mg_stokes_apply", self.handle, x.ptr, y.ptr)
 
@@ -232,7 +238,8 @@ def stokes_rhs_lifted(dims, coords, eta, bc, g, f):
     return bu, -yp
 
 
-def rift_hierarchy(c, nlevels, eta, opt: MGOptions, coarsen="qp-arithmetic", inner_its=1, inner_rtol=0.0):
+def rift_hierarchy(c, nlevels, eta, opt: MGOptions, coarsen="qp-arithmetic", inner_its=1, inner_rtol=0.0,
+                   schur="mass"):
     """GMG hierarchy of the rift saddle point (free-slip sides and base,
     x-face extension on every level) with the saddle-point preconditioner."""
     mg = GMG(c["mx"], c["my"], c["mz"], nlevels)
@@ -243,6 +250,8 @@ def rift_hierarchy(c, nlevels, eta, opt: MGOptions, coarsen="qp-arithmetic", inn
         mg.set_dirichlet(l, F.rift_bc(*d)[0])
     mg.setup(opt)
     mg.stokes_setup(inner_its, inner_rtol)
+    if schur != "mass":
+        mg.set_schur(schur)
     return mg
 
 

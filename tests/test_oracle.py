@@ -451,3 +451,27 @@ def test_point_projection_at_quadrature_points_is_qp_projection():
                  (m[0] * m[1] * m[2], 1))
     el = np.repeat(np.arange(m[0] * m[1] * m[2]), 27)
     assert rel_inf(O.q1_nodal_from_points(*m, el, qx, eta), O.q1_nodal_from_qp(*m, eta)) <= 1e-14
+
+
+def test_schur_bdb_blocks_brute_force():
+    """Element blocks of -B^T D^-1 B against the dense product of the
+    assembled A12 (one oracle product per pressure unit vector) and the
+    diagonal, Dirichlet rows dropped; every block is the negated SPD inverse."""
+    m = (3, 2, 2)
+    rng = np.random.default_rng(1)
+    coords = F.perturb_interior(F.box_coords(*m), *m, 0.1, rng)
+    E = m[0] * m[1] * m[2]
+    eta = F.eta_loguniform(E, 1e-2, 1e2, rng)
+    bc = F.dirichlet_all_faces(*m)
+    Minv = O.schur_bdb_blocks(*m, coords, eta, bc)
+    B = np.stack([O.a12_matmult(*m, coords, bc, np.eye(4 * E)[k]) for k in range(4 * E)], 1)
+    d = O.a11_diagonal(*m, coords, eta, bc)
+    fr = ~bc.astype(bool)
+    assert np.abs(B[~fr]).max() == 0.0
+    S = B[fr].T @ (B[fr] / d[fr, None])
+    for e in range(E):
+        blk = S[4 * e:4 * e + 4, 4 * e:4 * e + 4]
+        assert np.allclose(np.linalg.inv(-blk), Minv[e], rtol=1e-12, atol=0)
+        assert np.linalg.eigvalsh(-Minv[e]).min() > 0
+    # the assembled B^T D^-1 B couples neighbours; the blocks are its diagonal
+    assert np.abs(S[0:4, 4:8]).max() > 0
