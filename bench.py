@@ -78,6 +78,7 @@ def main() -> int:
     ap.add_argument("--stokes", type=int, nargs=3, metavar=("MX", "MY", "MZ"), default=None,
                     help="run the rift saddle-point leg on this box (e.g. 192 192 96)")
     ap.add_argument("--stokes-maxit", type=int, default=600)
+    ap.add_argument("--stokes-restart", type=int, default=40)
     ap.add_argument("--stokes-schur", choices=("mass", "bdb"), default="mass",
                     help="Schur approximation of the rift leg: scaled viscous mass or B^T diag(A11)^-1 B blocks")
     args = ap.parse_args()
@@ -235,7 +236,7 @@ def main() -> int:
     if args.batch_solve > 0:
         out["batch_solve"] = batch_solve_leg(args.batch_solve)
     if args.stokes:
-        out["stokes"] = stokes_leg(tuple(args.stokes), args.stokes_maxit, args.stokes_schur)
+        out["stokes"] = stokes_leg(tuple(args.stokes), args.stokes_maxit, args.stokes_schur, args.stokes_restart)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -461,12 +462,12 @@ def solve_leg(m: int) -> dict:
     return res
 
 
-def stokes_leg(dims, maxit: int, schur: str = "mass") -> dict:
+def stokes_leg(dims, maxit: int, schur: str = "mass", restart: int = 40) -> dict:
     """Rift saddle point: a 1e3 crust (z > 0.75) over a 1e0 mantle with a
     seeded 1e-3 notch, power-law n = 3 viscosity of a seeded extension
     velocity at the quadrature points (clipped to [1e-3, 1e3]), gravity load;
     free-slip sides and base, extension velocity on the x-faces, free top.
-    FGMRES(40) in FP64 on [u | p] with the fused saddle-point MatMult and the
+    FGMRES(restart) in FP64 on [u | p] with the fused saddle-point MatMult and the
     upper block-triangular preconditioner (Schur approximation: the scaled
     viscous mass, or element blocks of B^T diag(A11)^-1 B; one 5-level
     V-cycle for A11: Chebyshev(4)-Jacobi, FP32 or FP64).  Reports the fused MatMult rate, then both solves to 1e-8 or maxit."""
@@ -488,7 +489,7 @@ def stokes_leg(dims, maxit: int, schur: str = "mass") -> dict:
                        "power-law n=3 viscosity of a seeded extension velocity (contrast 1e6), gravity, "
                        "free-slip sides and base, x-face extension, free top",
            "velocity_dofs": nu, "pressure_dofs": 4 * E, "levels": nl,
-           "krylov": "FGMRES(40), FP64, fused saddle-point MatMult",
+           "krylov": f"FGMRES({restart}), FP64, fused saddle-point MatMult",
            "preconditioner": "upper block-triangular: " + {"mass": "scaled viscous-mass",
                                                             "bdb": "B^T diag(A11)^-1 B element-block"}[schur]
                              + " Schur, one V-cycle for A11"}
@@ -517,7 +518,7 @@ def stokes_leg(dims, maxit: int, schur: str = "mass") -> dict:
         mg = rift_hierarchy(c, nl, eta, default_options(smoother_precision=prec, pre_its=4, post_its=4,
                                                           esteig=(0.0, 0.1, 0.0, 1.1)), schur=schur)
         ts = time.perf_counter() - t0
-        u, p, its, rn, hist, secs = stokes_solve_timed(mg, bu, bp, rtol=1e-8, maxit=maxit, restart=40)
+        u, p, its, rn, hist, secs = stokes_solve_timed(mg, bu, bp, rtol=1e-8, maxit=maxit, restart=restart)
         mg.destroy()
         sol[tag] = (u, p)
         res[tag] = {"its": its, "seconds": round(secs, 3), "ms_per_it": round(1e3 * secs / max(its, 1), 2),
@@ -529,6 +530,8 @@ def stokes_leg(dims, maxit: int, schur: str = "mass") -> dict:
         "u": float(np.linalg.norm(sol["fp32"][0] - sol["fp64"][0]) / np.linalg.norm(sol["fp64"][0])),
         "p": float(np.linalg.norm(sol["fp32"][1] - sol["fp64"][1]) / np.linalg.norm(sol["fp64"][1]))}
     res["speedup_fp32_vs_fp64_per_it"] = round(res["fp64"]["ms_per_it"] / res["fp32"]["ms_per_it"], 3)
+    if res["fp32"]["converged"] and res["fp64"]["converged"]:
+        res["speedup_fp32_vs_fp64_time_to_1e-8"] = round(res["fp64"]["seconds"] / res["fp32"]["seconds"], 3)
     return res
 
 
