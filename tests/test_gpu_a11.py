@@ -11,7 +11,7 @@ from ptatin3d_amd import fields as F
 from ptatin3d_amd._lib import DeviceArray, PtmfError
 from ptatin3d_amd.operators import MatA11MF, MatStokesMF
 
-from conftest import rel_inf
+from conftest import complete_nodes as _complete, rel_inf, sub_elems as _sub_elems, sub_nodes as _sub_nodes
 
 pytestmark = pytest.mark.gpu
 TOL64 = 1e-12
@@ -247,3 +247,71 @@ def test_reference_golden_values_viscous_sinker(gpu, case):
         y = A.a12_host(F.reference_pressure_test_vector(*m))
         assert _close9(y @ y, case["A12"]), y @ y
     A.destroy()
+
+
+# -- full-size parity through sub-boxes ----------------------------------------
+# Without Dirichlet masking, a node strictly inside a sub-box of elements (or
+# on a domain face the sub-box touches) receives contributions only from the
+# sub-box's elements, so the full-size GPU product must equal the oracle's
+# product on the sub-box alone there.  This checks the benchmark sizes
+# against the oracle at the cost of a few thousand oracle elements.
+
+def _subbox_a11(m, coords, eta, x, y, lo, n):
+    sc = _sub_nodes(m, coords, lo, n).reshape(-1)
+    se = _sub_elems(m, eta, lo, n, 27).reshape(-1)
+    sx = _sub_nodes(m, x, lo, n).reshape(-1)
+    yo = O.a11_tensor(*n, sc, se, sx).reshape(2 * n[2] + 1, 2 * n[1] + 1, 2 * n[0] + 1, 3)
+    ok = _complete(m, lo, n)
+    return _sub_nodes(m, y, lo, n)[ok], yo[ok], (sc, se, sx)
+
+
+def test_config3_full_size_subbox_parity(gpu):
+    """128^3, contrast-1e6 layered/ellipsoid viscosity on a perturbed mesh:
+    the full apply against the oracle on an interior and a corner sub-box,
+    plus symmetry and linearity at full size."""
+    c = F.config3()
+    m = (c["mx"], c["my"], c["mz"])
+    rng = np.random.default_rng(17)
+    coords = F.perturb_interior(c["coords"], *m, 0.1, rng)
+    eta = c["eta"]
+    A = _op(m, coords, eta, np.zeros_like(c["bc"]))
+    x = rng.uniform(-1, 1, coords.size)
+    y = A.mult_host(x)
+    for lo, n in (((50, 61, 70), (10, 9, 8)), ((0, 0, 120), (8, 7, 8))):
+        yg, yo, _ = _subbox_a11(m, coords, eta, x, y, lo, n)
+        assert rel_inf(yg, yo) <= TOL64, (lo, n)
+    z = rng.uniform(-1, 1, coords.size)
+    yz = A.mult_host(z)
+    assert abs(z @ y - x @ yz) <= 1e-12 * np.sqrt((z @ z) * (y @ y))
+    y2 = A.mult_host(x + 2.0 * z)
+    assert rel_inf(y2, y + 2.0 * yz) <= 1e-12
+    A.destroy()
+
+
+def test_config5_full_size_subbox_parity(gpu):
+    """192x192x96 rift (power-law viscosity of the seeded extension velocity,
+    contrast 1e6): the fused saddle-point product [A11 A12; A21 0] against the
+    oracle on an interior and a corner sub-box; the A21 rows of the sub-box
+    elements are element-local and compared in full."""
+    from ptatin3d_amd.multigrid import powerlaw_viscosity
+
+    c = F.config5()
+    m = (c["mx"], c["my"], c["mz"])
+    eta = powerlaw_viscosity(m, c["coords"], c["u0"], c["B"], c["nexp"], c["eta_min"], c["eta_max"])
+    assert eta.min() == 1e-3 and eta.max() == 1e3
+    rng = np.random.default_rng(23)
+    u = rng.uniform(-1, 1, c["coords"].size)
+    E = m[0] * m[1] * m[2]
+    p = rng.uniform(-1, 1, 4 * E)
+    S = MatStokesMF(*m).setup(c["coords"], eta, np.zeros_like(c["bc"]))
+    yu, yp = S.stokes_host(u, p)
+    S.destroy()
+    for lo, n in (((90, 37, 70), (9, 10, 8)), ((0, 0, 0), (8, 8, 7))):
+        zero = np.zeros(3 * F.nnodes(*n), np.uint8)
+        yg, yo, (sc, se, su) = _subbox_a11(m, c["coords"], eta, u, yu, lo, n)
+        sp = _sub_elems(m, p, lo, n, 4).reshape(-1)
+        ya12 = O.a12_matmult(*n, sc, zero, sp).reshape(2 * n[2] + 1, 2 * n[1] + 1, 2 * n[0] + 1, 3)
+        yo = yo + ya12[_complete(m, lo, n)]
+        assert rel_inf(yg, yo) <= TOL64, (lo, n)
+        ypo = O.a21_matmult(*n, sc, zero, su)
+        assert rel_inf(_sub_elems(m, yp, lo, n, 4).reshape(-1), ypo) <= TOL64, (lo, n)

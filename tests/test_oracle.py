@@ -475,3 +475,36 @@ def test_schur_bdb_blocks_brute_force():
         assert np.linalg.eigvalsh(-Minv[e]).min() > 0
     # the assembled B^T D^-1 B couples neighbours; the blocks are its diagonal
     assert np.abs(S[0:4, 4:8]).max() > 0
+
+
+def test_subbox_locality_of_the_unmasked_products():
+    """The property the full-size GPU parity tests rest on: without Dirichlet
+    masking, the A11 and A12 sums at the complete nodes of a sub-box (strictly
+    inside it, or on a domain face it touches) and the A21 rows of its
+    elements depend only on the sub-box's elements."""
+    from conftest import complete_nodes, sub_elems, sub_nodes
+
+    m = (9, 8, 7)
+    rng = np.random.default_rng(31)
+    coords = F.perturb_interior(F.box_coords(*m), *m, 0.1, rng)
+    E = m[0] * m[1] * m[2]
+    eta = F.eta_loguniform(E, 1e-3, 1e3, rng)
+    u = rng.uniform(-1, 1, coords.size)
+    p = rng.uniform(-1, 1, 4 * E)
+    zf = np.zeros(coords.size, np.uint8)
+    yu = O.a11_tensor(*m, coords, eta, u) + O.a12_matmult(*m, coords, zf, p)
+    yp = O.a21_matmult(*m, coords, zf, u)
+    for lo, n in (((3, 2, 2), (4, 3, 3)), ((0, 0, 0), (3, 4, 2)), ((6, 5, 4), (3, 3, 3))):
+        sc = sub_nodes(m, coords, lo, n).reshape(-1)
+        su = sub_nodes(m, u, lo, n).reshape(-1)
+        se = sub_elems(m, eta, lo, n, 27).reshape(-1)
+        sp = sub_elems(m, p, lo, n, 4).reshape(-1)
+        zs = np.zeros(sc.size, np.uint8)
+        ys = (O.a11_tensor(*n, sc, se, su) + O.a12_matmult(*n, sc, zs, sp)).reshape(
+            2 * n[2] + 1, 2 * n[1] + 1, 2 * n[0] + 1, 3)
+        ok = complete_nodes(m, lo, n)
+        assert rel_inf(sub_nodes(m, yu, lo, n)[ok], ys[ok]) <= 1e-13
+        # the incomplete nodes do differ (the check is not vacuous)
+        if not ok.all():
+            assert np.abs(sub_nodes(m, yu, lo, n)[~ok] - ys[~ok]).max() > 1e-6
+        assert rel_inf(sub_elems(m, yp, lo, n, 4).reshape(-1), O.a21_matmult(*n, sc, zs, su)) <= 1e-13
