@@ -52,8 +52,9 @@ __global__ void k_axpby(long n, T a, const T *__restrict__ x, T b, T *__restrict
     y[i] = a * x[i] + b * y[i];
 }
 
+// r = b - Ax; r may alias Ax (elementwise, same index)
 template <typename T>
-__global__ void k_residual(long n, const T *__restrict__ b, const T *__restrict__ Ax, T *__restrict__ r) {
+__global__ void k_residual(long n, const T *__restrict__ b, const T *Ax, T *r) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     r[i] = b[i] - Ax[i];
 }
