@@ -518,10 +518,23 @@ def stokes_leg(dims, maxit: int, schur: str = "mass", restart: int = 40) -> dict
         mg = rift_hierarchy(c, nl, eta, default_options(smoother_precision=prec, pre_its=4, post_its=4,
                                                           esteig=(0.0, 0.1, 0.0, 1.1)), schur=schur)
         ts = time.perf_counter() - t0
+        # the V-cycle alone (the part the smoother precision changes)
+        db, dv = DeviceArray.from_host(bu), DeviceArray(nu)
+        mg.vcycle(db, dv)
+        call("ptmf_device_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(5):
+            mg.vcycle(db, dv)
+        call("ptmf_device_synchronize")
+        vms = 1e3 * (time.perf_counter() - t0) / 5
+        db.free()
+        dv.free()
         u, p, its, rn, hist, secs = stokes_solve_timed(mg, bu, bp, rtol=1e-8, maxit=maxit, restart=restart)
         mg.destroy()
         sol[tag] = (u, p)
         res[tag] = {"its": its, "seconds": round(secs, 3), "ms_per_it": round(1e3 * secs / max(its, 1), 2),
+                    "vcycle_ms": round(vms, 2),
+                    "outer_ms_per_it": round(1e3 * secs / max(its, 1) - vms, 2),
                     "setup_seconds": round(ts, 3), "rel_residual": float(rn / bn),
                     "converged": bool(rn <= 1e-8 * bn)}
         print(f"# stokes leg {tag}: {its} its, {secs:.2f} s, rel residual {rn / bn:.2e}",
