@@ -305,7 +305,9 @@ int ptmf_bmg_fcg(ptmf_bmg bmg, const double *const *b, double *const *x, double 
 int ptmf_mg_stokes_setup(ptmf_mg mg, int inner_its, double inner_rtol);
 /* Schur approximation: 0 scaled viscous mass (the reference's, default);
  * 1 element blocks of B^T D^-1 B, D the A11 diagonal (no counterpart in the
- * reference). */
+ * reference).  Type 1 returns PTMF_ERR_ARG, with the mass blocks kept, when
+ * an element has fewer than 4 independent free velocity rows (its block is
+ * singular). */
 int ptmf_mg_stokes_set_schur(ptmf_mg mg, int type);
 int ptmf_mg_stokes_apply(ptmf_mg mg, const double *x, double *y);
 int ptmf_mg_stokes_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, int restart,

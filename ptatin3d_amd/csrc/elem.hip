@@ -473,7 +473,8 @@ __global__ void __launch_bounds__(64)
 // -B^T A^-1 B that sees the stiffness of the neighbours through D.
 __global__ void __launch_bounds__(64)
     schur_bdb_kernel(int mx, int my, long nel, long X, long XY, const double *__restrict__ xc,
-                     const uint8_t *__restrict__ bcn, const double *__restrict__ dg, double *__restrict__ Minv) {
+                     const uint8_t *__restrict__ bcn, const double *__restrict__ dg, double *__restrict__ Minv,
+                     int *__restrict__ singular) {
   __shared__ double sx[kEPB][27][3];
   __shared__ double sG[kEPB][27][14];  // G (9), -w detJ, NIp (4)
   __shared__ double sr[kEPB][32][10];
@@ -543,10 +544,15 @@ __global__ void __launch_bounds__(64)
         k++;
       }
     double B[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    // S_e has rank <= the element's free velocity rows: a vanishing pivot
+    // (fewer than 4 independent free rows) is reported, not inverted silently
+    double amax = 0.0;
+    for (int c = 0; c < 4; c++) amax = fmax(amax, fabs(A[c][c]));
     for (int c = 0; c < 4; c++) {
       int pr = c;
       for (int r = c + 1; r < 4; r++)
         if (fabs(A[r][c]) > fabs(A[pr][c])) pr = r;
+      if (!(fabs(A[pr][c]) > 1e-13 * amax)) atomicOr(singular, 1);
       for (int j = 0; j < 4; j++) {
         const double ta = A[c][j], tb = B[c][j];
         A[c][j] = A[pr][j]; B[c][j] = B[pr][j];
@@ -674,12 +680,13 @@ int launch_schur_blocks(const ptmf_op_s *op, double *Me, double *Minv) {
   return PTMF_OK;
 }
 
-int launch_schur_bdb(const ptmf_op_s *op, const double *dg, double *Minv) {
+int launch_schur_bdb(const ptmf_op_s *op, const double *dg, double *Minv, int *singular) {
   PTMF_TRY(init_constants());
   const Box &b = op->box;
   const long nblk = (b.nel + kEPB - 1) / kEPB;
   schur_bdb_kernel<<<dim3((unsigned)nblk), dim3(64), 0, op->stream>>>(b.mx, b.my, b.nel, b.X, b.X * b.Y,
-                                                                      op->d64.coords, op->bcnode, dg, Minv);
+                                                                      op->d64.coords, op->bcnode, dg, Minv,
+                                                                      singular);
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }

@@ -1911,7 +1911,26 @@ int ptmf_mg_stokes_set_schur(ptmf_mg mg, int type) {
     // before the element sums; stmp is free outside a preconditioner
     // application
     PTMF_TRY(ptmf_a11_diagonal(F.op, mg->stmp));
-    PTMF_TRY(ptmf::launch_schur_bdb(F.op, mg->stmp, mg->sMinv));
+    int *dflag = nullptr, flag = 0;
+    PTMF_TRY(ptmf::dalloc((void **)&dflag, sizeof(int)));
+    int st = PTMF_OK;
+    if (hipMemsetAsync(dflag, 0, sizeof(int), mg->stream) != hipSuccess)
+      st = ptmf_set_error(PTMF_ERR_DEVICE, "mg_stokes_set_schur: memset");
+    if (st == PTMF_OK) st = ptmf::launch_schur_bdb(F.op, mg->stmp, mg->sMinv, dflag);
+    if (st == PTMF_OK && hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, mg->stream) != hipSuccess)
+      st = ptmf_set_error(PTMF_ERR_DEVICE, "mg_stokes_set_schur: flag copy");
+    if (st == PTMF_OK && hipStreamSynchronize(mg->stream) != hipSuccess)
+      st = ptmf_set_error(PTMF_ERR_DEVICE, "mg_stokes_set_schur: synchronize");
+    (void)hipFree(dflag);
+    if (st != PTMF_OK) return st;
+    if (flag) {
+      // keep a usable preconditioner: back to the viscous-mass blocks
+      PTMF_TRY(ptmf::launch_schur_blocks(F.op, nullptr, mg->sMinv));
+      PTMF_HIP(hipStreamSynchronize(mg->stream));
+      return ptmf_set_error(PTMF_ERR_ARG,
+                            "mg_stokes_set_schur: a B^T D^-1 B element block is singular (an element with fewer "
+                            "than 4 independent free velocity rows); the viscous-mass blocks stay in use");
+    }
   }
   PTMF_HIP(hipStreamSynchronize(mg->stream));
   return PTMF_OK;

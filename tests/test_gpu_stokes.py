@@ -184,3 +184,35 @@ def test_rift_bdb_schur_converges(gpu):
     assert rn <= 1e-8 * bn and its <= 60, its
     yu, yp = _oracle_stokes(dims, c["coords"], eta, c["bc"], u, p)
     assert np.linalg.norm(np.concatenate([bu - yu, bp - yp])) <= 2e-8 * bn
+
+
+def test_bdb_schur_singular_block_is_an_error(gpu):
+    """2x2x2 box with every velocity dof Dirichlet except the x component of
+    the centre node: each element has one free velocity row, so its
+    B^T D^-1 B block (rank <= 1) is singular.  set_schur reports it and
+    leaves the viscous-mass blocks in place."""
+    from ptatin3d_amd._lib import PtmfError
+
+    m = (2, 2, 2)
+    coords, eta, _, rng = _case(m)
+    bc = np.ones(3 * F.nnodes(*m), np.uint8)
+    bc[3 * (2 + 5 * (2 + 5 * 2))] = 0
+    mg = GMG(*m, 2)
+    mg.set_coordinates(coords)
+    mg.set_viscosity(1, eta)
+    mg.coarsen_viscosity()
+    mg.set_dirichlet(1, bc)
+    mg.set_dirichlet(0, F.dirichlet_all_faces(1, 1, 1))
+    mg.setup(default_options(smoother_precision=64))
+    mg.stokes_setup()
+    with pytest.raises(PtmfError, match="singular"):
+        mg.set_schur("bdb")
+    E = 8
+    rp = rng.uniform(-1, 1, 4 * E)
+    bu = np.zeros(3 * F.nnodes(*m))
+    u, p, its, rn, hist, secs = stokes_solve_timed(mg, bu, rp, rtol=1e-30, maxit=1, restart=5)
+    want = np.einsum("eij,ej->ei", np.linalg.inv(O.schur_mass_blocks(*m, coords, eta)),
+                     rp.reshape(E, 4)).reshape(-1)
+    alpha = (p @ want) / (want @ want)
+    assert np.abs(p).max() > 0 and np.abs(p - alpha * want).max() <= 1e-11 * np.abs(p).max()
+    mg.destroy()
