@@ -13,10 +13,15 @@ the layered 1e6-contrast workload (FP32 V-cycle vs FP64 V-cycle, flexible CG
 in FP64, both to a 1e-8 relative FP64 residual), at --solve-m elements per
 direction (default 64; --no-solve skips it).  --copy-probe runs one plain
 device copy of 256 MiB after the timed region (the reference kernel for HBM
-byte counters).
+byte counters).  Opt-in legs: --batch-solve N (batched multigrid solve of N
+boxes) and --stokes MX MY MZ (the rift saddle point: fused MatMult rate, then
+FP32- and FP64-V-cycle FGMRES solves to 1e-8; --stokes-schur mass|bdb,
+--stokes-restart, --stokes-maxit).
 
   python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--m 64] [--no-cpu]
                   [--solve-m 64 | --no-solve] [--copy-probe]
+                  [--batch-solve 256] [--stokes 192 192 96 [--stokes-schur bdb]
+                  [--stokes-restart 100] [--stokes-maxit 1200]]
 """
 from __future__ import annotations
 
@@ -78,7 +83,7 @@ def main() -> int:
     ap.add_argument("--stokes", type=int, nargs=3, metavar=("MX", "MY", "MZ"), default=None,
                     help="run the rift saddle-point leg on this box (e.g. 192 192 96)")
     ap.add_argument("--stokes-maxit", type=int, default=600)
-    ap.add_argument("--stokes-restart", type=int, default=40)
+    ap.add_argument("--stokes-restart", type=int, default=40, help="FGMRES restart length of the rift leg")
     ap.add_argument("--stokes-schur", choices=("mass", "bdb"), default="mass",
                     help="Schur approximation of the rift leg: scaled viscous mass or B^T diag(A11)^-1 B blocks")
     args = ap.parse_args()
