@@ -430,6 +430,7 @@ def solve_leg(m: int) -> dict:
     both are solved to ||b - A x|| <= 1e-8 ||b|| (FP64 residual of the FP64
     operator), and each solution is compared with a 1e-11 FP64 solve."""
     from ptatin3d_amd import fields as F
+    from ptatin3d_amd._lib import DeviceArray, call
     from ptatin3d_amd.multigrid import GMG, default_options, solve_timed
 
     c = F.config3(m)
@@ -453,10 +454,22 @@ def solve_leg(m: int) -> dict:
         mg.setup(default_options(smoother_precision=prec, pre_its=10, post_its=10, galerkin_levels=gal,
                                  esteig=(0.0, 0.02, 0.0, 1.1)))
         ts = time.perf_counter() - t0
+        # the V-cycle alone (the part the smoother precision changes)
+        db, dv = DeviceArray.from_host(c["b"]), DeviceArray(c["b"].size)
+        mg.vcycle(db, dv)
+        call("ptmf_device_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(5):
+            mg.vcycle(db, dv)
+        call("ptmf_device_synchronize")
+        vms = 1e3 * (time.perf_counter() - t0) / 5
+        db.free()
+        dv.free()
         x, its, rn, hist, secs = solve_timed(mg, c["b"], rtol=rtol, maxit=4000, krylov="fcg")
         runs[tag] = x
         res[tag] = {"its": its, "seconds": round(secs, 4), "setup_seconds": round(ts, 3),
-                    "rel_residual": float(rn / bnorm)}
+                    "rel_residual": float(rn / bnorm), "vcycle_ms": round(vms, 3),
+                    "outer_ms_per_it": round(1e3 * secs / max(its, 1) - vms, 3)}
         mg.destroy()
         print(f"# solve leg {tag}: {its} its, {secs:.2f} s, rel residual {rn / bnorm:.2e}",
               file=sys.stderr, flush=True)
