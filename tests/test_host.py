@@ -1,0 +1,53 @@
+"""Host-side logic that runs without a GPU: the autotuner's candidate set, the
+rift workload's boundary data, and the benchmark's one-GPU contract."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+from ptatin3d_amd import autotune
+from ptatin3d_amd import fields as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_autotune_candidates_respect_the_kernel_limits():
+    """Every candidate fits the tile kernel: at most 21 elements per wave,
+    a node window of at most 130 nodes, and no tile larger than the box."""
+    for m in ((64, 64, 64), (5, 4, 3), (1, 1, 1), (22, 1, 3), (192, 192, 96)):
+        c = autotune.candidates(*m)
+        assert c
+        for TX, TY, ZC in c:
+            assert TX * TY <= 21 and (2 * TX + 1) * (2 * TY + 1) <= 130
+            assert 1 <= TX <= m[0] and 1 <= TY <= m[1] and 1 <= ZC <= m[2]
+
+
+def test_rift_boundary_data():
+    """Extension on the x-faces (u_x = -1 at x = 0, +1 at x = 2), free slip
+    on the y-faces (u_y = 0) and the base (u_z = 0), free top."""
+    mx, my, mz = 3, 2, 2
+    X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+    m, g = F.rift_bc(mx, my, mz)
+    m = m.reshape(Z, Y, X, 3).astype(bool)
+    g = g.reshape(Z, Y, X, 3)
+    assert m[..., 0].sum() == 2 * Y * Z and m[:, :, 0, 0].all() and m[:, :, -1, 0].all()
+    assert m[..., 1].sum() == 2 * X * Z and m[:, 0, :, 1].all() and m[:, -1, :, 1].all()
+    assert m[..., 2].sum() == X * Y and m[0, :, :, 2].all() and not m[-1, :, :, 2].any()
+    assert (g[:, :, 0, 0] == -1.0).all() and (g[:, :, -1, 0] == 1.0).all()
+    assert np.count_nonzero(g) == 2 * Y * Z and not g[~m].any()
+    # the load is gravity only, denser mantle below the crust
+    c = F.config5(mx, my, mz)
+    f = c["f"].reshape(-1, 3)
+    z = c["coords"].reshape(-1, 3)[:, 2]
+    assert not f[:, :2].any() and (f[:, 2] < 0).all()
+    assert f[z < 0.75, 2].min() < f[z > 0.75, 2].max()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode != 0
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in line["error"] and line["n_gpus"] == 2
