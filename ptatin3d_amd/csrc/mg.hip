@@ -599,11 +599,16 @@ __global__ void k_divide(long n, const double *__restrict__ a, const double *__r
 // consecutive nodes of one offset row (coalesced), and there are enough
 // waves to cover memory latency on the small coarse levels; the 25 partial
 // sums are reduced through LDS.
+// kCheb: the Chebyshev-Jacobi step is applied to the product as it is
+// formed, pkp1 = os dinv (b - A x) + om1 pkm1 + om x (x = p_k), and y is not
+// written (the same arithmetic as the product followed by k_cheb_step).
 constexpr int kStNodes = 16;
-template <typename T>
+template <typename T, bool kCheb = false>
 __global__ void __launch_bounds__(25 * kStNodes)
     k_stencil_apply(long X, long Y, long Z, const T *__restrict__ S, const T *__restrict__ x,
-                    T *__restrict__ y) {
+                    T *__restrict__ y, const T *__restrict__ b = nullptr, const T *__restrict__ dinv = nullptr,
+                    const T *__restrict__ pkm1 = nullptr, T os = T(0), T om1 = T(0), T om = T(0),
+                    T *__restrict__ pkp1 = nullptr) {
   __shared__ T red[25][kStNodes][3];
   const long N = X * Y * Z;
   const int t = threadIdx.x;
@@ -636,7 +641,13 @@ __global__ void __launch_bounds__(25 * kStNodes)
   if (g < 3 && m < N) {
     T acc = 0;
     for (int q = 0; q < 25; q++) acc += red[q][ml][g];
-    y[3 * m + g] = acc;
+    const long i = 3 * m + g;
+    if constexpr (kCheb) {
+      const T z = (b[i] - acc) * dinv[i];
+      pkp1[i] = os * z + om1 * pkm1[i] + om * x[i];
+    } else {
+      y[i] = acc;
+    }
   }
 }
 
@@ -935,9 +946,18 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
   for (int i = 0; i < its; i++) {
     c[kp1] = 2.0 * mu * c[k] - c[km1];
     const double omega = omegaprod * c[k] / c[kp1];
-    PTMF_TRY(matmult<T>(L, p[k], r));
-    k_cheb_step<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, dinv, p[km1], p[k], (T)(omega * scale),
-                                               (T)(1.0 - omega), (T)omega, p[kp1]);
+    if (L.galerkin) {
+      // stencil product and Chebyshev update in one launch
+      const Box &bx = L.op->box;
+      const T *S = sizeof(T) == 8 ? (const T *)L.S64 : (const T *)L.S32;
+      const long nb = (bx.nnodes + kStNodes - 1) / kStNodes;
+      k_stencil_apply<T, true><<<dim3((unsigned)nb), dim3(25 * kStNodes), 0, L.op->stream>>>(
+          bx.X, bx.Y, bx.Z, S, p[k], r, b, dinv, p[km1], (T)(omega * scale), (T)(1.0 - omega), (T)omega, p[kp1]);
+    } else {
+      PTMF_TRY(matmult<T>(L, p[k], r));
+      k_cheb_step<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, dinv, p[km1], p[k], (T)(omega * scale),
+                                                 (T)(1.0 - omega), (T)omega, p[kp1]);
+    }
     PTMF_HIP(hipGetLastError());
     const int t = km1;
     km1 = k; k = kp1; kp1 = t;
