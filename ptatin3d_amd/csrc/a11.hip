@@ -1233,7 +1233,7 @@ int ensure_pa(ptmf_op_s *op) {
 }
 
 template <typename T>
-int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, bool faces) {
+int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, int faces) {
   const OpData<T> &d = *(const OpData<T> *)(sizeof(T) == 8 ? (const void *)&op->d64
                                                            : (const void *)&op->d32);
   const Tiling &tl = op->tiling;
@@ -1263,7 +1263,7 @@ int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, bool faces) {
   if (faces) {
     a11_face_kernel<T><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(a, fbx, fby, fbz, y);
     PTMF_HIP(hipGetLastError());
-    PTMF_TRY(launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
+    if (faces == 1) PTMF_TRY(launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
   }
   return PTMF_OK;
 }
@@ -1288,8 +1288,8 @@ int launch_stokes_mult(ptmf_op_s *op, const double *x, double *y) {
 
 template int launch_a11<double>(const ptmf_op_s *, const double *, double *, bool);
 template int launch_a11<float>(const ptmf_op_s *, const float *, float *, bool);
-template int launch_a11_mult<double>(ptmf_op_s *, const double *, double *, bool);
-template int launch_a11_mult<float>(ptmf_op_s *, const float *, float *, bool);
+template int launch_a11_mult<double>(ptmf_op_s *, const double *, double *, int);
+template int launch_a11_mult<float>(ptmf_op_s *, const float *, float *, int);
 
 }  // namespace ptmf
 

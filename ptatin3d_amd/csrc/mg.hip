@@ -90,13 +90,21 @@ __global__ void k_cheb_first(long n, const T *__restrict__ b, const T *__restric
   }
 }
 
-// p_kp1 = (omega scale) dinv (b - A p_k) + (1 - omega) p_km1 + omega p_k
+// p_kp1 = (omega scale) dinv (b - A p_k) + (1 - omega) p_km1 + omega p_k;
+// with bcn (per-node Dirichlet bits) the Dirichlet rows of A p_k are taken as
+// p_k (the MatMult's y = x), so Apk may hold the element sums there
 template <typename T>
 __global__ void k_cheb_step(long n, const T *__restrict__ b, const T *__restrict__ Apk,
                             const T *__restrict__ dinv, const T *__restrict__ pkm1,
-                            const T *__restrict__ pk, T os, T om1, T om, T *__restrict__ pkp1) {
+                            const T *__restrict__ pk, T os, T om1, T om, T *__restrict__ pkp1,
+                            const uint8_t *__restrict__ bcn = nullptr) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const T z = (b[i] - Apk[i]) * dinv[i];
+    T ap = Apk[i];
+    if (bcn) {
+      const long nd = i / 3;
+      if ((bcn[nd] >> (int)(i - 3 * nd)) & 1u) ap = pk[i];
+    }
+    const T z = (b[i] - ap) * dinv[i];
     pkp1[i] = os * z + om1 * pkm1[i] + om * pk[i];
   }
 }
@@ -954,9 +962,10 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
       k_stencil_apply<T, true><<<dim3((unsigned)nb), dim3(25 * kStNodes), 0, L.op->stream>>>(
           bx.X, bx.Y, bx.Z, S, p[k], r, b, dinv, p[km1], (T)(omega * scale), (T)(1.0 - omega), (T)omega, p[kp1]);
     } else {
-      PTMF_TRY(matmult<T>(L, p[k], r));
+      // MatMult without its Dirichlet-copy launch; the update takes y = x there
+      PTMF_TRY(launch_a11_mult<T>(L.op, p[k], r, 2));
       k_cheb_step<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, dinv, p[km1], p[k], (T)(omega * scale),
-                                                 (T)(1.0 - omega), (T)omega, p[kp1]);
+                                                 (T)(1.0 - omega), (T)omega, p[kp1], L.op->bcnode);
     }
     PTMF_HIP(hipGetLastError());
     const int t = km1;

@@ -134,9 +134,11 @@ namespace ptmf {
 template <typename T>
 int launch_a11(const ptmf_op_s *op, const T *u, T *y, bool mask_columns);
 // MatMult_MFStokes_A11: y = A11 mask(x) on free rows, y = x on Dirichlet rows.
-// faces == false runs the element (tile) kernel alone, for timing.
+// faces: 1 the whole MatMult; 0 the element (tile) kernel alone, for timing;
+// 2 without the Dirichlet rows (left as the element sums), for a consumer
+// that takes y = x there itself.
 template <typename T>
-int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, bool faces);
+int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, int faces);
 template <typename T>
 int launch_fill(T *x, T v, long n, hipStream_t s);
 template <typename T>
