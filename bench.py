@@ -202,6 +202,9 @@ def main() -> int:
     if not args.no_cpu:
         from oracle import oracle as O
 
+        # compiled here, for this host's CPU (the library in the tree is a
+        # portable build)
+        native = O.use_native_build()
         threads = O.num_threads()
         t0 = time.perf_counter()
         nrun = 0
@@ -219,7 +222,8 @@ def main() -> int:
             "cores": threads,
             "kind": "port",
             "sample": f"{nrun} FP64 A11 MatMults of the same {m}^3 workload "
-                      f"({el:.1f} s) on {threads} OpenMP threads, {cpu_model()}",
+                      f"({el:.1f} s) on {threads} OpenMP threads, {cpu_model()}, "
+                      f"{'-march=native on this host' if native else 'portable x86-64-v3 build'}",
         }
         err = float(np.abs(y_gpu - y_cpu).max() / np.abs(y_cpu).max())
         out["parity_rel_inf_vs_oracle"] = err

@@ -29,7 +29,8 @@
  *
  * Unless the name ends in _host, vector arguments are DEVICE pointers
  * (allocate with ptmf_malloc).  All work is ordered on the operator's
- * stream; entry points that return host-visible results synchronize it.
+ * stream; entry points that return host-visible results synchronize it, and
+ * the ptmf_memcpy_* / ptmf_memset helpers wait for the whole device first.
  * Every function returns PTMF_OK (0) or an error code; ptmf_last_error()
  * gives the message.
  */
@@ -82,8 +83,10 @@ typedef struct {
                              point Jacobi lets elements cut by viscosity jumps stretch it) */
   int use_graph;          /* 1: replay the V-cycle as a captured HIP graph from its second call
                              (needs levels_mg == nlevels; a new ptmf_mg_setup or
-                             ptmf_mg_set_level_eigs recaptures; changing a level operator's
-                             tiling or kernel variant after set-up needs ptmf_mg_setup) */
+                             ptmf_mg_set_level_eigs recaptures, and so does a change of a level
+                             operator's coefficients, tiling, kernel variant or Dirichlet list
+                             through ptmf_mg_level_op: the next cycle runs eagerly, the one
+                             after it is captured again) */
   int transfer;           /* 0: DMDA Q1 interpolation on the Q2 node lattice (the reference's
                              DMCreateInterpolation, ptatin_driver_linear_ts.c:989); 1: Q2
                              embedding (fine nodal values of the coarse Q2 function) */
@@ -159,12 +162,20 @@ int ptmf_op_state(ptmf_op op, unsigned long *state);
  * (stokes_operators.h:59; MFStokesWrapper_A11_Tensor, stokes_operators_tensor.c:181). */
 int ptmf_a11_spmv(ptmf_op op, const double *u, double *y);
 /* Same contract on HOST arrays (for MFStokesWrapper_A11_HIP in a PETSc
- * adapter): copies u in, adds A u into Yu on the host. */
+ * adapter): copies u in and writes Yu = A11_local u.  The reference's caller
+ * zeroes Yu right before the plugin call (stokes_operators.c:1181), so this
+ * is its Yu += A u; like the CUDA backend (stokes_operators_cuda.cu:453) the
+ * old contents of Yu are not read, which saves one PCIe transfer. */
 int ptmf_a11_spmv_host(ptmf_op op, const double *u, double *Yu);
-/* MatMult_MFStokes_A11: y = A11 * mask(x) on free rows, y = x on Dirichlet rows. */
+/* MatMult_MFStokes_A11: y = A11 * mask(x) on free rows, y = x on Dirichlet rows.
+ * Asynchronous: returns once the work is enqueued on the operator's stream;
+ * later work of the same operator, the ptmf_memcpy_* / ptmf_memset helpers
+ * and ptmf_op_synchronize are ordered after it. */
 int ptmf_a11_mult(ptmf_op op, const double *x, double *y);
 /* FP32 version of ptmf_a11_mult (FP32 coordinates, viscosity and accumulation). */
 int ptmf_a11_mult_f32(ptmf_op op, const float *x, float *y);
+/* wait for all work enqueued on the operator's stream */
+int ptmf_op_synchronize(ptmf_op op);
 /* MatGetDiagonal_MFStokes_A11: diag(A11), 1 in Dirichlet slots. */
 int ptmf_a11_diagonal(ptmf_op op, double *d);
 /* l1 row sums of the A11 MatMult operator (device vector 3N): for each

@@ -32,6 +32,21 @@ def load() -> C.CDLL:
     return _lib
 
 
+def use_native_build() -> bool:
+    """Compile the oracle for this host's CPU (-march=native) and make every
+    later call use it; for the CPU-baseline timing, on the host that times
+    it.  Returns False (portable build kept) when no compiler is available.
+    Call before any other oracle function."""
+    global _lib
+    native = os.path.join(HERE, "liboracle_native.so")
+    r = subprocess.run(["make", "-s", "-B", "-C", HERE, "native"], capture_output=True, text=True)
+    if r.returncode != 0 or not os.path.exists(native):
+        return False
+    _lib = C.CDLL(native)
+    _lib.orc_num_threads.restype = C.c_int
+    return True
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 

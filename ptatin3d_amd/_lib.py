@@ -60,6 +60,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_a11_spmv_host": (_I, [_P, _P, _P]),
     "ptmf_a11_mult": (_I, [_P, _P, _P]),
     "ptmf_a11_mult_f32": (_I, [_P, _P, _P]),
+    "ptmf_op_synchronize": (_I, [_P]),
     "ptmf_a11_diagonal": (_I, [_P, _P]),
     "ptmf_a11_l1_diagonal": (_I, [_P, _P]),
     "ptmf_a12_mult": (_I, [_P, _P, _P]),
@@ -208,6 +209,20 @@ class DeviceArray:
                 _lib.ptmf_free(self.ptr)
         except Exception:
             pass
+
+
+def require(where: str, n: int, dtype, *arrays) -> None:
+    """Refuse device arrays that the C function would index past their end or
+    read in the wrong precision: each must be a live DeviceArray of exactly n
+    elements of dtype (PTMF_ERR_ARG, before anything reaches the GPU)."""
+    dt = np.dtype(dtype)
+    for a in arrays:
+        if not isinstance(a, DeviceArray) or a.ptr is None or not a.ptr.value:
+            raise PtmfError(1, where, "expected a live DeviceArray")
+        if a.dtype != dt:
+            raise PtmfError(1, where, f"array of {a.dtype}, expected {dt}")
+        if a.n != int(n):
+            raise PtmfError(1, where, f"array of {a.n} elements, expected {int(n)}")
 
 
 def _hp(a: np.ndarray) -> C.c_void_p:

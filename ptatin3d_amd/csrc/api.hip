@@ -101,26 +101,35 @@ int ptmf_free(void *dptr) {
   return PTMF_OK;
 }
 
+// The helpers below first wait for all work on the device, so they are
+// ordered after asynchronous entry points on any operator's stream.
 int ptmf_memcpy_htod(void *dst, const void *src, size_t bytes) {
-  if (bytes) PTMF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  if (!bytes) return PTMF_OK;
+  PTMF_HIP(hipDeviceSynchronize());
+  PTMF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return PTMF_OK;
 }
 
 int ptmf_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
-  if (bytes) PTMF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  if (!bytes) return PTMF_OK;
+  PTMF_HIP(hipDeviceSynchronize());
+  PTMF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return PTMF_OK;
 }
 
 int ptmf_memcpy_dtod(void *dst, const void *src, size_t bytes) {
   PTMF_CHECK(bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0, PTMF_ERR_ARG,
              "memcpy_dtod: 16-byte aligned pointers and sizes only");
+  PTMF_HIP(hipDeviceSynchronize());
   PTMF_TRY(ptmf::launch_copy16(src, dst, (long)(bytes / 16), 0));
   PTMF_HIP(hipStreamSynchronize(0));
   return PTMF_OK;
 }
 
 int ptmf_memset(void *dptr, int value, size_t bytes) {
-  if (bytes) PTMF_HIP(hipMemset(dptr, value, bytes));
+  if (!bytes) return PTMF_OK;
+  PTMF_HIP(hipDeviceSynchronize());
+  PTMF_HIP(hipMemset(dptr, value, bytes));
   return PTMF_OK;
 }
 
@@ -376,22 +385,26 @@ int ptmf_a11_spmv_host(ptmf_op op, const double *u, double *Yu) {
   const long n3 = 3 * op->box.nnodes;
   if (!op->host_stage) PTMF_TRY(ptmf_malloc((void **)&op->host_stage, sizeof(double) * n3));
   if (!op->host_stage2) PTMF_TRY(ptmf_malloc((void **)&op->host_stage2, sizeof(double) * n3));
+  // Yu is written, not uploaded and added to: the caller zeroes it right
+  // before the plugin call (stokes_operators.c:1181), and the reference's
+  // CUDA backend likewise overwrites it with the device result
+  // (stokes_operators_cuda.cu:453), so only u crosses PCIe on the way in
   PTMF_HIP(hipMemcpyAsync(op->host_stage, u, sizeof(double) * n3, hipMemcpyHostToDevice, op->stream));
-  PTMF_HIP(hipMemcpyAsync(op->host_stage2, Yu, sizeof(double) * n3, hipMemcpyHostToDevice, op->stream));
+  PTMF_TRY(ptmf::launch_fill<double>(op->host_stage2, 0.0, n3, op->stream));
   PTMF_TRY(ptmf::launch_a11<double>(op, op->host_stage, op->host_stage2, false));
   PTMF_HIP(hipMemcpyAsync(Yu, op->host_stage2, sizeof(double) * n3, hipMemcpyDeviceToHost, op->stream));
   PTMF_HIP(hipStreamSynchronize(op->stream));
   return PTMF_OK;
 }
 
-int ptmf_a11_mult(ptmf_op op, const double *x, double *y) {
-  PTMF_TRY(a11_mult<double>(op, x, y));
-  PTMF_HIP(hipStreamSynchronize(op->stream));
-  return PTMF_OK;
-}
+// Asynchronous on the operator's stream: a Krylov loop that keeps its vectors
+// on the device pays no host stall per MatMult.
+int ptmf_a11_mult(ptmf_op op, const double *x, double *y) { return a11_mult<double>(op, x, y); }
 
-int ptmf_a11_mult_f32(ptmf_op op, const float *x, float *y) {
-  PTMF_TRY(a11_mult<float>(op, x, y));
+int ptmf_a11_mult_f32(ptmf_op op, const float *x, float *y) { return a11_mult<float>(op, x, y); }
+
+int ptmf_op_synchronize(ptmf_op op) {
+  PTMF_CHECK(op, PTMF_ERR_ARG, "op_synchronize: null operator");
   PTMF_HIP(hipStreamSynchronize(op->stream));
   return PTMF_OK;
 }

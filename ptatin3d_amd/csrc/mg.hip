@@ -65,6 +65,59 @@ __global__ void k_cast(long n, const Ts *__restrict__ x, Td *__restrict__ y) {
     y[i] = (Td)x[i];
 }
 
+template <typename T>
+__global__ void k_copy(long n, const T *__restrict__ x, T *__restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+
+// Dense column-major product y = A x (A: n x m, leading dimension lda) in
+// two launches: each work-group forms the partial sums of kGvRows rows over
+// one chunk of kGvCols columns (x chunk staged in LDS, consecutive threads
+// read consecutive rows of a column), then the chunks are summed in a fixed
+// order, so the result is deterministic and no atomics are needed.
+constexpr int kGvRows = 256;
+constexpr int kGvCols = 512;
+
+__global__ void __launch_bounds__(kGvRows)
+    k_gemv_part(long n, long m, const double *__restrict__ A, long lda, const double *__restrict__ x,
+                double *__restrict__ part) {
+  __shared__ double xs[kGvCols];
+  const long j0 = (long)blockIdx.y * kGvCols;
+  const int nj = (int)(m - j0 < kGvCols ? m - j0 : kGvCols);
+  for (int t = threadIdx.x; t < nj; t += kGvRows) xs[t] = x[j0 + t];
+  __syncthreads();
+  const long i = (long)blockIdx.x * kGvRows + threadIdx.x;
+  if (i >= n) return;
+  const double *a = A + i + j0 * lda;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int j = 0;
+  for (; j + 4 <= nj; j += 4) {
+    s0 += a[(long)j * lda] * xs[j];
+    s1 += a[(long)(j + 1) * lda] * xs[j + 1];
+    s2 += a[(long)(j + 2) * lda] * xs[j + 2];
+    s3 += a[(long)(j + 3) * lda] * xs[j + 3];
+  }
+  for (; j < nj; j++) s0 += a[(long)j * lda] * xs[j];
+  part[(long)blockIdx.y * n + i] = (s0 + s1) + (s2 + s3);
+}
+
+__global__ void k_gemv_sum(long n, int nchunks, const double *__restrict__ part, double *__restrict__ y) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < nchunks; c++) s += part[(long)c * n + i];
+    y[i] = s;
+  }
+}
+
+// mirror the lower triangle of a column-major n x n matrix into its upper one
+__global__ void k_symmetrize(long n, double *__restrict__ A) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n * n; t += (long)gridDim.x * blockDim.x) {
+    const long i = t % n, j = t / n;
+    if (i < j) A[i + j * n] = A[j + i * n];
+  }
+}
+
 // y += (Td) x
 template <typename Ts, typename Td>
 __global__ void k_add_cast(long n, const Ts *__restrict__ x, Td *__restrict__ y) {
@@ -794,7 +847,8 @@ struct ptmf_mg_s {
   double *coef_dev = nullptr, *coef_host = nullptr;  // Krylov coefficients (pinned upload)
   // dense coarse factor
   rocblas_handle blas = nullptr;
-  double *Adense = nullptr;
+  double *Adense = nullptr;     // explicit inverse, full symmetric (column-major)
+  double *dense_part = nullptr; // partial sums of the inverse's product
   int ndense = 0;
   int *dinfo = nullptr;
   // Krylov workspaces (finest level, FP64): velocity block and saddle point
@@ -809,6 +863,15 @@ struct ptmf_mg_s {
   hipGraphExec_t vexec = nullptr;
   double *gb = nullptr, *gx = nullptr;
   int vcalls = 0;
+  // what each level's MatMult nodes were captured with: a later change of a
+  // level operator's coefficients, tiling, kernel variant or Dirichlet list
+  // (through ptmf_mg_level_op) makes the graph stale, and it is dropped
+  struct GraphKey {
+    unsigned long state = 0;
+    int TX = 0, TY = 0, ZC = 0, variant = 0;
+    const void *faces = nullptr, *bcidx = nullptr, *et64 = nullptr, *et32 = nullptr;
+    long nbc = 0;
+  } gkey[16];
   int s_inner_its = 1;       // A11^-1 in the preconditioner: 1 = one V-cycle, k > 1 = up to k
   double s_inner_rtol = 0.0; // flexible-CG iterations to relative tolerance s_inner_rtol
   double *fcg[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // flexible CG workspace
@@ -938,8 +1001,10 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
   double c[3];
   // p[km1] = x, p[k] = first step
   int km1 = 0, k = 1, kp1 = 2;
+  // copies and fills are kernels: the V-cycle may be captured into a graph,
+  // which then holds kernel nodes only
   if (nonzero)
-    PTMF_HIP(hipMemcpyAsync(p[km1], x, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+    k_copy<T><<<nblocks(n), kBS, 0, s>>>(n, x, p[km1]);
   else
     PTMF_TRY(launch_fill<T>(p[km1], T(0), n, s));  // a kernel: the V-cycle may be captured
   if (nonzero) {
@@ -971,7 +1036,8 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
     const int t = km1;
     km1 = k; k = kp1; kp1 = t;
   }
-  PTMF_HIP(hipMemcpyAsync(x, p[k], sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+  k_copy<T><<<nblocks(n), kBS, 0, s>>>(n, p[k], x);
+  PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }
 
@@ -980,10 +1046,12 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
 // Cholesky factor): one symmetric matrix-vector product instead of two
 // latency-bound triangular solves per V-cycle
 int dense_solve(ptmf_mg_s *mg, const double *b, double *x) {
-  const int n = mg->ndense;
-  const double one = 1.0, zero = 0.0;
-  rocblas_status st = rocblas_dsymv(mg->blas, rocblas_fill_lower, n, &one, mg->Adense, n, b, 1, &zero, x, 1);
-  PTMF_CHECK(st == rocblas_status_success, PTMF_ERR_DEVICE, "rocblas_dsymv: status %d", (int)st);
+  const long n = mg->ndense;
+  const int nchunks = (int)((n + kGvCols - 1) / kGvCols);
+  const dim3 grid((unsigned)((n + kGvRows - 1) / kGvRows), (unsigned)nchunks);
+  k_gemv_part<<<grid, kGvRows, 0, mg->stream>>>(n, n, mg->Adense, n, b, mg->dense_part);
+  k_gemv_sum<<<nblocks(n), kBS, 0, mg->stream>>>(n, nchunks, mg->dense_part, x);
+  PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }
 
@@ -1234,6 +1302,13 @@ int assemble_dense(ptmf_mg_s *mg) {
   PTMF_HIP(hipMemcpyAsync(&info, mg->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
   PTMF_HIP(hipStreamSynchronize(s));
   PTMF_CHECK(info == 0, PTMF_ERR_DIVERGED, "coarse inverse failed (info %d)", info);
+  // full symmetric inverse for the project's own product kernel (the V-cycle
+  // graph then holds no library nodes); partial sums buffer of that product
+  k_symmetrize<<<nblocks(n * n), kBS, 0, s>>>(n, mg->Adense);
+  PTMF_HIP(hipGetLastError());
+  if (!mg->dense_part)
+    PTMF_TRY(dalloc((void **)&mg->dense_part, sizeof(double) * n * ((n + kGvCols - 1) / kGvCols)));
+  PTMF_HIP(hipStreamSynchronize(s));
   return PTMF_OK;
 }
 
@@ -1505,6 +1580,7 @@ int ptmf_mg_destroy(ptmf_mg *pmg) {
   (void)hipFree(mg->coef_dev);
   if (mg->coef_host) (void)hipHostFree(mg->coef_host);
   (void)hipFree(mg->Adense);
+  (void)hipFree(mg->dense_part);
   (void)hipFree(mg->dinfo);
   if (mg->red_host) (void)hipHostFree(mg->red_host);
   if (mg->blas) rocblas_destroy_handle(mg->blas);
@@ -1720,6 +1796,21 @@ int vcycle_body(ptmf_mg_s *mg, const double *b, double *x) {
   return cast<float, double>(mg->stream, F.n, F.x32, x);
 }
 
+ptmf_mg_s::GraphKey graph_key(const ptmf_op_s *op) {
+  ptmf_mg_s::GraphKey k;
+  k.state = op->state;
+  k.TX = op->tiling.TX;
+  k.TY = op->tiling.TY;
+  k.ZC = op->tiling.ZC;
+  k.variant = op->variant;
+  k.faces = op->faces;
+  k.bcidx = op->bcidx;
+  k.et64 = op->et64;
+  k.et32 = op->et32;
+  k.nbc = op->nbc;
+  return k;
+}
+
 // Capture one V-cycle from gb into gx as a HIP graph.  Everything the cycle
 // builds lazily (tile-ordered data) exists after the first, eager call.
 int capture_vcycle(ptmf_mg_s *mg) {
@@ -1741,7 +1832,18 @@ int capture_vcycle(ptmf_mg_s *mg) {
   }
   mg->vgraph = g;
   mg->vexec = ex;
+  for (int l = 0; l < mg->L; l++) mg->gkey[l] = graph_key(mg->lev[l].op);
   return PTMF_OK;
+}
+
+bool graph_stale(const ptmf_mg_s *mg) {
+  for (int l = 0; l < mg->L; l++) {
+    const ptmf_mg_s::GraphKey k = graph_key(mg->lev[l].op), &c = mg->gkey[l];
+    if (k.state != c.state || k.TX != c.TX || k.TY != c.TY || k.ZC != c.ZC || k.variant != c.variant ||
+        k.faces != c.faces || k.bcidx != c.bcidx || k.et64 != c.et64 || k.et32 != c.et32 || k.nbc != c.nbc)
+      return true;
+  }
+  return false;
 }
 }  // namespace
 }  // namespace ptmf
@@ -1753,6 +1855,10 @@ int ptmf_mg_vcycle(ptmf_mg mg, const double *b, double *x) {
   PTMF_CHECK(mg && b && x, PTMF_ERR_ARG, "mg_vcycle: null");
   PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_vcycle: not set up");
   const bool graph = mg->opt.use_graph && mg->opt.levels_mg == mg->L;
+  // a level operator changed since the capture: replay nothing stale, run
+  // the next cycle eagerly (it rebuilds the operator's derived data) and
+  // capture again after it
+  if (graph && mg->vexec && ptmf::graph_stale(mg)) ptmf::drop_graph(mg);
   if (!graph || mg->vcalls++ == 0) return ptmf::vcycle_body(mg, b, x);
   if (!mg->vexec) PTMF_TRY(ptmf::capture_vcycle(mg));
   const long n = mg->lev[mg->L - 1].n;

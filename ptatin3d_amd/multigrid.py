@@ -15,7 +15,7 @@ import time
 import numpy as np
 
 from . import _lib, fields as F
-from ._lib import DeviceArray, MGOptions, call
+from ._lib import DeviceArray, MGOptions, call, require
 
 
 def default_options(**kw) -> MGOptions:
@@ -39,6 +39,13 @@ class GMG:
         self.dims = [(mx >> (nlevels - 1 - l), my >> (nlevels - 1 - l), mz >> (nlevels - 1 - l))
                      for l in range(nlevels)]
         self.n = 3 * F.nnodes(mx, my, mz)
+        self.nstokes = self.n + 4 * mx * my * mz
+
+    def level_n(self, level: int) -> int:
+        """velocity dofs of a level (3 * nodes)"""
+        if not 0 <= int(level) < self.nlevels:
+            raise _lib.PtmfError(1, "GMG", f"level {level} outside 0..{self.nlevels - 1}")
+        return 3 * F.nnodes(*self.dims[int(level)])
 
     def level_coordinates(self, fine_coords: np.ndarray) -> list[np.ndarray]:
         """Injected coordinates of every level (for host-side field evaluation)."""
@@ -93,23 +100,31 @@ class GMG:
 
     def level_apply(self, level: int, x: DeviceArray, y: DeviceArray) -> None:
         prec = 32 if x.dtype == np.float32 else 64
+        require("GMG.level_apply", self.level_n(level), x.dtype, x, y)
         call("ptmf_mg_level_apply", self.handle, int(level), prec, x.ptr, y.ptr)
 
     def vcycle(self, b: DeviceArray, x: DeviceArray) -> None:
+        require("GMG.vcycle", self.n, np.float64, b, x)
         call("ptmf_mg_vcycle", self.handle, b.ptr, x.ptr)
 
     def prolong_add(self, level: int, xc: DeviceArray, yf: DeviceArray) -> None:
+        require("GMG.prolong_add", self.level_n(level - 1), np.float64, xc)
+        require("GMG.prolong_add", self.level_n(level), np.float64, yf)
         call("ptmf_mg_prolong_add", self.handle, int(level), xc.ptr, yf.ptr)
 
     def restrict(self, level: int, xf: DeviceArray, yc: DeviceArray) -> None:
+        require("GMG.restrict", self.level_n(level), np.float64, xf)
+        require("GMG.restrict", self.level_n(level - 1), np.float64, yc)
         call("ptmf_mg_restrict", self.handle, int(level), xf.ptr, yc.ptr)
 
     def smooth(self, level: int, b: DeviceArray, x: DeviceArray, its: int, nonzero_guess: bool) -> None:
         prec = 32 if b.dtype == np.float32 else 64
+        require("GMG.smooth", self.level_n(level), b.dtype, b, x)
         call("ptmf_mg_smooth", self.handle, int(level), prec, int(its), int(nonzero_guess), b.ptr, x.ptr)
 
     def fgmres(self, b: DeviceArray, x: DeviceArray, rtol=1e-8, maxit=200, restart=30,
                history_len=512):
+        require("GMG.fgmres", self.n, np.float64, b, x)
         its = C.c_int(0)
         rn = C.c_double(0)
         hist = np.zeros(history_len)
@@ -120,6 +135,7 @@ class GMG:
 
     def fcg(self, b: DeviceArray, x: DeviceArray, rtol=1e-8, maxit=500, history_len=1024):
         """Flexible CG preconditioned by the V-cycle, from the given x."""
+        require("GMG.fcg", self.n, np.float64, b, x)
         its = C.c_int(0)
         rn = C.c_double(0)
         hist = np.zeros(history_len)
@@ -141,10 +157,12 @@ class GMG:
         call("ptmf_mg_stokes_set_schur", self.handle, {"mass": 0, "bdb": 1}[kind])
 
     def stokes_apply(self, x: DeviceArray, y: DeviceArray) -> None:
+        require("GMG.stokes_apply", self.nstokes, np.float64, x, y)
         call("ptmf_mg_stokes_apply", self.handle, x.ptr, y.ptr)
 
     def stokes_fgmres(self, b: DeviceArray, x: DeviceArray, rtol=1e-8, maxit=500, restart=50,
                       history_len=1024):
+        require("GMG.stokes_fgmres", self.nstokes, np.float64, b, x)
         its = C.c_int(0)
         rn = C.c_double(0)
         hist = np.zeros(history_len)
@@ -271,10 +289,18 @@ class BatchGMG:
     def _ptrs(arrs):
         return (C.c_void_p * len(arrs))(*[a.ptr for a in arrs])
 
+    def _check(self, bs, xs):
+        if len(bs) != len(self.mgs) or len(xs) != len(self.mgs):
+            raise _lib.PtmfError(1, "BatchGMG", f"{len(bs)}/{len(xs)} arrays for {len(self.mgs)} boxes")
+        for g, b, x in zip(self.mgs, bs, xs):
+            require("BatchGMG", g.n, np.float64, b, x)
+
     def vcycle(self, bs, xs) -> None:
+        self._check(bs, xs)
         call("ptmf_bmg_vcycle", self.handle, self._ptrs(bs), self._ptrs(xs))
 
     def fcg(self, bs, xs, rtol=1e-8, maxit=500):
+        self._check(bs, xs)
         n = len(self.mgs)
         its = (C.c_int * n)()
         rn = (C.c_double * n)()

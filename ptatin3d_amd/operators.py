@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import DeviceArray, call
+from ._lib import DeviceArray, call, require
 
 A11_OPERATOR_TYPES = ("hip",)
 
@@ -84,34 +84,47 @@ class MatA11MF:
     # -- operations on device arrays --------------------------------------
     def mult(self, x: DeviceArray, y: DeviceArray) -> None:
         """MatMult_MFStokes_A11 (stokes_operators.c:1142-1205)."""
+        require("MatA11MF.mult", 3 * self.nnodes, x.dtype, x, y)
         if x.dtype == np.float32:
             call("ptmf_a11_mult_f32", self.handle, x.ptr, y.ptr)
         else:
             call("ptmf_a11_mult", self.handle, x.ptr, y.ptr)
+        # the C entry point is asynchronous; other operators run on their own
+        # streams, so the Python host waits before y is used elsewhere
+        call("ptmf_op_synchronize", self.handle)
 
     def spmv(self, u: DeviceArray, y: DeviceArray) -> None:
         """SpMVOp_MatMult: y += A11_local u (no boundary handling)."""
+        require("MatA11MF.spmv", 3 * self.nnodes, np.float64, u, y)
         call("ptmf_a11_spmv", self.handle, u.ptr, y.ptr)
 
     def get_diagonal(self, d: DeviceArray) -> None:
         """MatGetDiagonal_MFStokes_A11 (stokes_operators.c:1060-1099)."""
+        require("MatA11MF.get_diagonal", 3 * self.nnodes, np.float64, d)
         call("ptmf_a11_diagonal", self.handle, d.ptr)
 
     def powerlaw_viscosity(self, u: DeviceArray, B: DeviceArray, nexp: float, eta_min: float,
                            eta_max: float, eta: DeviceArray) -> None:
         """Quadrature-point power-law viscosity of velocity u (element-major 27E)."""
+        require("MatA11MF.powerlaw_viscosity", 3 * self.nnodes, np.float64, u)
+        require("MatA11MF.powerlaw_viscosity", 27 * self.nel, np.float64, B, eta)
         call("ptmf_op_powerlaw_viscosity", self.handle, u.ptr, B.ptr, float(nexp), float(eta_min),
              float(eta_max), eta.ptr)
 
     def project_points(self, elem: DeviceArray, xi: DeviceArray, val: DeviceArray, eta: DeviceArray) -> None:
         """Material-point values to the quadrature points (element-major 27E)."""
+        require("MatA11MF.project_points", val.n, np.int64, elem)
+        require("MatA11MF.project_points", 3 * val.n, np.float64, xi)
+        require("MatA11MF.project_points", 27 * self.nel, np.float64, eta)
         call("ptmf_op_project_points", self.handle, int(val.n), elem.ptr, xi.ptr, val.ptr, eta.ptr)
 
     def set_viscosity_device(self, eta: DeviceArray) -> None:
+        require("MatA11MF.set_viscosity_device", 27 * self.nel, np.float64, eta)
         call("ptmf_op_set_viscosity_device", self.handle, eta.ptr)
 
     def get_l1_diagonal(self, d: DeviceArray) -> None:
         """l1 row sums of the MatMult operator (the l1-Jacobi scaling)."""
+        require("MatA11MF.get_l1_diagonal", 3 * self.nnodes, np.float64, d)
         call("ptmf_a11_l1_diagonal", self.handle, d.ptr)
 
     # -- host conveniences --------------------------------------------------
@@ -127,8 +140,11 @@ class MatA11MF:
 
     def spmv_host(self, u: np.ndarray, Yu: np.ndarray) -> None:
         """MFStokesWrapper_A11_HIP contract on host arrays: Yu += A u."""
-        u = np.ascontiguousarray(u, dtype=np.float64)
-        assert Yu.dtype == np.float64 and Yu.flags.c_contiguous
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        if u.size != 3 * self.nnodes or Yu.size != 3 * self.nnodes:
+            raise _lib.PtmfError(1, "MatA11MF.spmv_host", f"{u.size}/{Yu.size} values, expected {3 * self.nnodes}")
+        if Yu.dtype != np.float64 or not Yu.flags.c_contiguous:
+            raise _lib.PtmfError(1, "MatA11MF.spmv_host", "Yu must be a contiguous float64 array")
         call("ptmf_a11_spmv_host", self.handle, _lib._hp(u), _lib._hp(Yu))
 
     def l1_diagonal_host(self) -> np.ndarray:
@@ -147,6 +163,7 @@ class MatA11MF:
 
     def time_mult(self, x: DeviceArray, y: DeviceArray, reps: int) -> float:
         """ms per MatMult over `reps` back-to-back applies (HIP events)."""
+        require("MatA11MF.time_mult", 3 * self.nnodes, x.dtype, x, y)
         prec = 32 if x.dtype == np.float32 else 64
         a = C.c_double(0)
         call("ptmf_a11_time", self.handle, prec, x.ptr, y.ptr, int(reps), C.byref(a))
@@ -154,6 +171,7 @@ class MatA11MF:
 
     def time_kernel(self, x: DeviceArray, y: DeviceArray, reps: int) -> float:
         """ms per launch of the element kernel alone (HIP events)."""
+        require("MatA11MF.time_kernel", 3 * self.nnodes, x.dtype, x, y)
         prec = 32 if x.dtype == np.float32 else 64
         a = C.c_double(0)
         call("ptmf_a11_time_kernel", self.handle, prec, x.ptr, y.ptr, int(reps), C.byref(a))
@@ -177,14 +195,19 @@ class MatStokesMF(MatA11MF):
 
     def mult_a12(self, p: DeviceArray, yu: DeviceArray) -> None:
         """MatMult_MFStokes_A12: yu = B p, Dirichlet velocity rows zeroed."""
+        require("MatStokesMF.mult_a12", 4 * self.nel, np.float64, p)
+        require("MatStokesMF.mult_a12", 3 * self.nnodes, np.float64, yu)
         call("ptmf_a12_mult", self.handle, p.ptr, yu.ptr)
 
     def mult_a21(self, u: DeviceArray, yp: DeviceArray) -> None:
         """MatMult_MFStokes_A21: yp = B^T mask(u)."""
+        require("MatStokesMF.mult_a21", 3 * self.nnodes, np.float64, u)
+        require("MatStokesMF.mult_a21", 4 * self.nel, np.float64, yp)
         call("ptmf_a21_mult", self.handle, u.ptr, yp.ptr)
 
     def mult_stokes(self, x: DeviceArray, y: DeviceArray) -> None:
         """MatMult_MFStokes_A: y = [A11 A12; A21 0] [u | p] in one pass."""
+        require("MatStokesMF.mult_stokes", 3 * self.nnodes + 4 * self.nel, np.float64, x, y)
         call("ptmf_stokes_mult", self.handle, x.ptr, y.ptr)
 
     def stokes_host(self, u: np.ndarray, p: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
@@ -240,10 +263,19 @@ class BatchA11:
     def _ptrs(arrs):
         return (C.c_void_p * len(arrs))(*[a.ptr for a in arrs])
 
+    def _check(self, xs, ys, precision):
+        if len(xs) != len(self.ops) or len(ys) != len(self.ops):
+            raise _lib.PtmfError(1, "BatchA11", f"{len(xs)}/{len(ys)} arrays for {len(self.ops)} boxes")
+        dt = np.float64 if precision == 64 else np.float32
+        for op, x, y in zip(self.ops, xs, ys):
+            require("BatchA11", 3 * op.nnodes, dt, x, y)
+
     def mult(self, xs, ys, precision: int = 64) -> None:
+        self._check(xs, ys, precision)
         call("ptmf_batch_mult", self.handle, int(precision), self._ptrs(xs), self._ptrs(ys))
 
     def time_mult(self, xs, ys, reps: int, precision: int = 64) -> float:
+        self._check(xs, ys, precision)
         ms = C.c_double()
         call("ptmf_batch_time", self.handle, int(precision), self._ptrs(xs), self._ptrs(ys), int(reps), C.byref(ms))
         return ms.value

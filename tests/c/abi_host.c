@@ -62,7 +62,7 @@ int main(int argc, char **argv) {
   CHECK(ptmf_a11_mult(op, dx, dy));
   CHECK(ptmf_memcpy_dtoh(y, dy, sizeof(double) * 3 * N));
   dump(argv[8], y, sizeof(double) * 3 * N);
-  /* SpMVOp_MatMult contract: Yu += A11_local u on host arrays, Yu pre-zeroed */
+  /* SpMVOp_MatMult contract on host arrays: Yu = A11_local u (the caller zeroes Yu first) */
   for (long i = 0; i < 3 * N; i++) y[i] = 0.0;
   CHECK(ptmf_a11_spmv_host(op, x, y));
   dump(argv[9], y, sizeof(double) * 3 * N);

@@ -257,6 +257,45 @@ def test_vcycle_graph_replays_match_eager(gpu, prec):
     mg_g.destroy()
 
 
+def test_vcycle_graph_follows_level_operator_changes(gpu):
+    """A level operator changed through ptmf_mg_level_op after the capture
+    (tiling: its face buffers are reallocated; viscosity: new coefficients
+    and a rebuilt tile-ordered copy) is never replayed stale: the graph
+    V-cycle keeps matching the eager one."""
+    import ctypes as C
+
+    from ptatin3d_amd._lib import call
+
+    m = (8, 8, 8)
+    mg_e, coords, eta = _hierarchy(m, 3, opt=default_options(smoother_precision=64, use_graph=0))
+    mg_g, _, _ = _hierarchy(m, 3, opt=default_options(smoother_precision=64, use_graph=1))
+    rng = np.random.default_rng(41)
+    n = 3 * F.nnodes(*m)
+    eta2 = eta * 10.0 ** rng.uniform(-1, 1, eta.size)
+
+    def change(mg, k):
+        h = mg.level_op_handle(2)
+        if k == 0:
+            call("ptmf_op_set_tiling", h, 1, 1, 1)
+        elif k == 1:
+            call("ptmf_op_set_viscosity", h, eta2.ctypes.data_as(C.c_void_p))
+        else:
+            call("ptmf_op_set_tiling", h, 7, 3, 3)
+
+    for k in range(3):
+        for rep in range(3):
+            b = rng.uniform(-1, 1, n)
+            db = DeviceArray.from_host(b)
+            xe, xg = DeviceArray(n), DeviceArray(n)
+            mg_e.vcycle(db, xe)
+            mg_g.vcycle(db, xg)
+            assert rel_inf(xg.download(), xe.download()) <= 1e-13, (k, rep)
+        change(mg_e, k)
+        change(mg_g, k)
+    mg_e.destroy()
+    mg_g.destroy()
+
+
 def _coarsen_numpy(dims_c, ef, mode):
     """Element / quadrature-point-local means of the children (modes 0-4)."""
     mxc, myc, mzc = dims_c

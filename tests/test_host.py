@@ -51,3 +51,72 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode != 0
     line = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in line["error"] and line["n_gpus"] == 2
+
+
+def _fake_array(n, dtype):
+    """A DeviceArray-shaped object that owns no device memory (its pointer is
+    never dereferenced: the wrappers must refuse it before any C call)."""
+    import ctypes as C
+
+    from ptatin3d_amd._lib import DeviceArray
+
+    class _NoFree(DeviceArray):
+        def __del__(self):
+            pass
+
+    a = _NoFree.__new__(_NoFree)
+    a.dtype = np.dtype(dtype)
+    a.n = int(n)
+    a.nbytes = a.n * a.dtype.itemsize
+    a.ptr = C.c_void_p(0x1000)
+    return a
+
+
+def test_wrappers_refuse_mismatched_device_arrays():
+    """Short arrays or mixed precisions never reach the kernels (they would
+    index past the allocation on the GPU): PTMF_ERR_ARG from the wrapper."""
+    import pytest
+
+    from ptatin3d_amd._lib import PtmfError
+    from ptatin3d_amd.multigrid import GMG
+    from ptatin3d_amd.operators import MatA11MF, MatStokesMF
+
+    A = MatStokesMF.__new__(MatStokesMF)
+    A.mx, A.my, A.mz = 3, 2, 2
+    A.nnodes, A.nel, A.handle = F.nnodes(3, 2, 2), 12, None
+    n = 3 * A.nnodes
+    bad = [
+        lambda: A.mult(_fake_array(n, np.float64), _fake_array(n - 3, np.float64)),
+        lambda: A.mult(_fake_array(n, np.float64), _fake_array(n, np.float32)),
+        lambda: A.mult(_fake_array(n, np.float32), _fake_array(n, np.float64)),
+        lambda: A.get_diagonal(_fake_array(n, np.float32)),
+        lambda: A.spmv(_fake_array(n + 3, np.float64), _fake_array(n, np.float64)),
+        lambda: A.mult_a12(_fake_array(4 * 12 - 1, np.float64), _fake_array(n, np.float64)),
+        lambda: A.mult_a21(_fake_array(n, np.float64), _fake_array(4 * 11, np.float64)),
+        lambda: A.mult_stokes(_fake_array(n + 48, np.float64), _fake_array(n, np.float64)),
+        lambda: A.spmv_host(np.zeros(n), np.zeros(n - 1)),
+        lambda: A.set_viscosity_device(_fake_array(27 * 12 - 1, np.float64)),
+    ]
+    g = GMG.__new__(GMG)
+    g.nlevels, g.dims, g.handle = 2, [(2, 1, 1), (4, 2, 2)], None
+    g.n = 3 * F.nnodes(4, 2, 2)
+    g.nstokes = g.n + 4 * 16
+    nc = 3 * F.nnodes(2, 1, 1)
+    bad += [
+        lambda: g.vcycle(_fake_array(g.n, np.float64), _fake_array(nc, np.float64)),
+        lambda: g.level_apply(0, _fake_array(nc, np.float32), _fake_array(nc, np.float64)),
+        lambda: g.level_apply(0, _fake_array(g.n, np.float64), _fake_array(g.n, np.float64)),
+        lambda: g.level_apply(2, _fake_array(g.n, np.float64), _fake_array(g.n, np.float64)),
+        lambda: g.smooth(1, _fake_array(g.n, np.float32), _fake_array(g.n, np.float64), 2, False),
+        lambda: g.prolong_add(1, _fake_array(g.n, np.float64), _fake_array(g.n, np.float64)),
+        lambda: g.restrict(1, _fake_array(g.n, np.float64), _fake_array(g.n, np.float64)),
+        lambda: g.fgmres(_fake_array(g.n, np.float64), _fake_array(g.n, np.float32)),
+        lambda: g.fcg(_fake_array(g.n - 1, np.float64), _fake_array(g.n, np.float64)),
+        lambda: g.stokes_fgmres(_fake_array(g.n, np.float64), _fake_array(g.n, np.float64)),
+        lambda: g.stokes_apply(_fake_array(g.nstokes, np.float64), _fake_array(g.n, np.float64)),
+    ]
+    for f in bad:
+        with pytest.raises(PtmfError) as ei:
+            f()
+        assert ei.value.code == 1
+    assert isinstance(A, MatA11MF)

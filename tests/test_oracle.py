@@ -453,6 +453,63 @@ def test_point_projection_at_quadrature_points_is_qp_projection():
     assert rel_inf(O.q1_nodal_from_points(*m, el, qx, eta), O.q1_nodal_from_qp(*m, eta)) <= 1e-14
 
 
+def _brute_schur_mass(m, coords, eta):
+    """element blocks -sum_q w detJ (1/eta_q) NIp_i NIp_j, numpy (P1GRel basis
+    at the physical quadrature point, geometry as in _brute_b)"""
+    nid = F.element_node_ids(*m)
+    E = m[0] * m[1] * m[2]
+    X = coords.reshape(-1, 3)
+    out = np.zeros((E, 4, 4))
+    for e in range(E):
+        xe = X[nid[e]]
+        avg = xe.mean(axis=0)
+        L = np.array([xe[14, 0] - xe[12, 0], xe[16, 1] - xe[10, 1], xe[22, 2] - xe[4, 2]])
+        for c in range(3):
+            for b in range(3):
+                for a in range(3):
+                    xi = (G3[a], G3[b], G3[c])
+                    n = [_n1(t) for t in xi]
+                    dn = [_dn1(t) for t in xi]
+                    Nv = np.einsum("k,j,i->kji", n[2], n[1], n[0]).reshape(27)
+                    gref = np.stack([
+                        np.einsum("k,j,i->kji", n[2], n[1], dn[0]).reshape(27),
+                        np.einsum("k,j,i->kji", n[2], dn[1], n[0]).reshape(27),
+                        np.einsum("k,j,i->kji", dn[2], n[1], n[0]).reshape(27)])
+                    w = W3[a] * W3[b] * W3[c] * np.linalg.det(gref @ xe)
+                    Np = np.concatenate([[1.0], (Nv @ xe - avg) / L])
+                    out[e] -= (w / eta[27 * e + a + 3 * b + 9 * c]) * np.outer(Np, Np)
+    return out
+
+
+def test_schur_mass_blocks_brute_force(curved):
+    """orc_schur_mass_blocks (stokes_assembly.c:386-409) against an independent
+    numpy quadrature of -int (1/eta) NIp_i NIp_j on curved elements with
+    quadrature-point viscosity; a flipped sign or eta in place of 1/eta fails."""
+    m, coords, eta, bc, rng = curved
+    Me = O.schur_mass_blocks(*m, coords, eta).reshape(-1, 4, 4)
+    ref = _brute_schur_mass(m, coords, eta)
+    assert np.abs(Me - ref).max() <= 1e-13 * np.abs(ref).max()
+    # negative definite, and the planted mistakes are caught
+    assert all(np.linalg.eigvalsh(-b).min() > 0 for b in Me)
+    assert np.abs(-Me - ref).max() > 1e-3 * np.abs(ref).max()
+    assert np.abs(_brute_schur_mass(m, coords, 1.0 / eta) - Me).max() > 1e-3 * np.abs(ref).max()
+
+
+def test_schur_mass_blocks_closed_form_affine():
+    """Axis-aligned bricks hx x hy x hz with constant eta: NIp = {1, (x-xc)/hx,
+    (y-yc)/hy, (z-zc)/hz}, so M_e = -(V/eta) diag(1, 1/12, 1/12, 1/12) exactly
+    (3-point Gauss integrates the quadratics)."""
+    m = (2, 3, 2)
+    coords = F.box_coords(*m, hi=(1.0, 2.0, 1.5))
+    E = m[0] * m[1] * m[2]
+    eta0 = 7.5
+    Me = O.schur_mass_blocks(*m, coords, np.full(27 * E, eta0)).reshape(-1, 4, 4)
+    V = (1.0 / 2) * (2.0 / 3) * (1.5 / 2)
+    ref = -(V / eta0) * np.diag([1.0, 1.0 / 12, 1.0 / 12, 1.0 / 12])
+    for e in range(E):
+        assert np.allclose(Me[e], ref, rtol=1e-13, atol=1e-16)
+
+
 def test_schur_bdb_blocks_brute_force():
     """Element blocks of -B^T D^-1 B against the dense product of the
     assembled A12 (one oracle product per pressure unit vector) and the
