@@ -63,7 +63,7 @@ typedef struct ptmf_bmg_s *ptmf_bmg;
  * pTatin3dStokesKSPConfigureFSGMG, ptatin_driver_linear_ts.c:819-882). */
 typedef struct {
   int levels_mg;          /* finest levels in the V-cycle (0: all); coarser levels form
-                             the coarse solver (FP64 PCG + FP64 V-cycle, dense at level 0) */
+                             the coarse solver (FP64 PCG + FP64 V-cycle, exact at level 0) */
   int smoother_precision; /* 32 or 64: operator, Chebyshev and transfers in the V-cycle */
   int pre_its, post_its;  /* Chebyshev iterations per smoothing (KSP max_it) */
   double esteig[4];       /* emin = e0*min + e1*max, emax = e2*min + e3*max of D^-1 A
@@ -90,6 +90,12 @@ typedef struct {
   int transfer;           /* 0: DMDA Q1 interpolation on the Q2 node lattice (the reference's
                              DMCreateInterpolation, ptatin_driver_linear_ts.c:989); 1: Q2
                              embedding (fine nodal values of the coarse Q2 function) */
+  int coarse_factor;      /* exact solve of level 0 (the reference assembles it,
+                             MatAssemble_StokesA_AUU, stokes_assembly.c:115-297, and factors it,
+                             -fieldsplit_u_mg_coarse_pc_type lu): 1 dense explicit inverse (at most
+                             32,768 dofs; needed by the batched multigrid); 2 sparse multifrontal
+                             Cholesky of the assembled level (nested-dissection order, any size);
+                             0 (default) dense up to 32,768 dofs, sparse above */
 } ptmf_mg_options;
 
 /* ---- library / device helpers ------------------------------------------ */
@@ -245,6 +251,10 @@ int ptmf_mg_coarsen_viscosity(ptmf_mg mg, int mode);
 int ptmf_mg_set_dirichlet(ptmf_mg mg, int level, const unsigned char *mask);
 /* diagonals, Jacobi scalings, Chebyshev eigenvalue bounds, coarse factor */
 int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt);
+/* the coarse factor of the last set-up: kind 1 dense / 2 sparse, its size in
+ * bytes, the depth of the nested-dissection tree (0 for dense) and the
+ * factorisation time in ms (sparse only; 0 for dense) */
+int ptmf_mg_coarse_info(ptmf_mg mg, int *kind, long *bytes, int *depths, double *setup_ms);
 int ptmf_mg_level_eigs(ptmf_mg mg, int level, double *emin, double *emax);
 int ptmf_mg_set_level_eigs(ptmf_mg mg, int level, double emin, double emax);
 /* one V-cycle on the finest level: x = M^-1 b (device FP64 arrays) */

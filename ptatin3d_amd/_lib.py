@@ -77,6 +77,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_mg_coarsen_viscosity": (_I, [_P, _I]),
     "ptmf_mg_set_dirichlet": (_I, [_P, _I, _P]),
     "ptmf_mg_setup": (_I, [_P, _P]),
+    "ptmf_mg_coarse_info": (_I, [_P, C.POINTER(_I), C.POINTER(_L), C.POINTER(_I), _PD]),
     "ptmf_mg_level_eigs": (_I, [_P, _I, _PD, _PD]),
     "ptmf_mg_set_level_eigs": (_I, [_P, _I, _D, _D]),
     "ptmf_mg_vcycle": (_I, [_P, _P, _P]),
@@ -120,6 +121,7 @@ class MGOptions(C.Structure):
         ("smoother_diag", C.c_int),
         ("use_graph", C.c_int),
         ("transfer", C.c_int),
+        ("coarse_factor", C.c_int),
     ]
 
 

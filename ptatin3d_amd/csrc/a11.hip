@@ -1232,16 +1232,17 @@ int ensure_pa(ptmf_op_s *op) {
   return PTMF_OK;
 }
 
+// The MatMult's kernels with the face buffers at fb (the operator's own, or
+// the static scratch of the allocation-free entry point).
 template <typename T>
-int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, int faces) {
+int launch_a11_mult_fb(ptmf_op_s *op, const T *x, T *y, int faces, T *fb) {
   const OpData<T> &d = *(const OpData<T> *)(sizeof(T) == 8 ? (const void *)&op->d64
                                                            : (const void *)&op->d32);
   const Tiling &tl = op->tiling;
   PTMF_CHECK(tl.TX * tl.TY <= kEPW && (2 * tl.TX + 1) * (2 * tl.TY + 1) <= kWinNodes && tl.ZC > 0,
              PTMF_ERR_ARG, "a11: invalid tiling %d x %d x %d", tl.TX, tl.TY, tl.ZC);
-  PTMF_CHECK(op->faces != nullptr, PTMF_ERR_NOT_SETUP, "a11: face buffers not allocated");
+  PTMF_CHECK(fb != nullptr, PTMF_ERR_NOT_SETUP, "a11: face buffers not allocated");
   const TileArgs a = tile_args(op);
-  T *fb = (T *)op->faces;
   const long ntiles = tl.ntiles;
   T *fbx = fb, *fby = fbx + 3 * a.fbx_n * ntiles, *fbz = fby + 3 * a.fby_n * ntiles;
   if (op->variant == 2) {
@@ -1266,6 +1267,11 @@ int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, int faces) {
     if (faces == 1) PTMF_TRY(launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
   }
   return PTMF_OK;
+}
+
+template <typename T>
+int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, int faces) {
+  return launch_a11_mult_fb<T>(op, x, y, faces, (T *)op->faces);
 }
 
 int launch_stokes_mult(ptmf_op_s *op, const double *x, double *y) {

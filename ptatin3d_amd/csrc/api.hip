@@ -158,7 +158,8 @@ int ptmf_op_create(int mx, int my, int mz, ptmf_op *out) {
   alloc((void **)&op->d32.eta, sizeof(float) * 27 * b.nel);
   alloc((void **)&op->bcnode, sizeof(uint8_t) * b.nnodes);
   if (st == PTMF_OK) {
-    hipError_t e = hipStreamCreate(&op->stream);
+    hipError_t e = hipGetDevice(&op->device);
+    if (e == hipSuccess) e = hipStreamCreate(&op->stream);
     if (e == hipSuccess) e = hipEventCreate(&op->ev0);
     if (e == hipSuccess) e = hipEventCreate(&op->ev1);
     if (e != hipSuccess) st = ptmf_set_error(PTMF_ERR_DEVICE, "op_create: %s", hipGetErrorString(e));

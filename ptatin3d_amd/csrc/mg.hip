@@ -35,6 +35,8 @@ namespace {
 constexpr int kBS = 256;
 constexpr int kRedBlocks = 1024;
 constexpr int kMaxDots = 512;
+constexpr int kRedSlots = 2 * kMaxDots + 4;  // reduction results per synchronisation
+constexpr long kDenseMaxDofs = 32768;  // largest level 0 of the dense coarse inverse
 
 inline unsigned nblocks(long n) {
   long g = (n + kBS - 1) / kBS;
@@ -276,6 +278,68 @@ __global__ void k_mupdate(long n, int k, const double *const *__restrict__ Z, co
     double s = x[i];
     for (int j = 0; j < k; j++) s += y[j] * Z[j][i];
     x[i] = s;
+  }
+}
+
+// One reduction pass of the low-synchronisation Gram-Schmidt step: for the
+// basis vectors V_0..V_k (chunk blockIdx.y holds V_8c..V_8c+7) the products
+// <V_i, V_k> and <V_i, w>, and <w, w>; slot i, k + 1 + i and 2k + 2 of part
+// (one value per block: part[slot * gridDim.x + block]).
+__global__ void k_lsgs_partial(long n, int k, const double *const *__restrict__ V, const double *__restrict__ w,
+                               double *__restrict__ part) {
+  __shared__ double red[17][kBS / 64];
+  const int i0 = 8 * blockIdx.y;
+  const int ni = min(8, k + 1 - i0);
+  const double *vk = V[k];
+  double aa[8] = {0, 0, 0, 0, 0, 0, 0, 0}, at[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ww = 0.0;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const double we = w[e], ve = vk[e];
+    if (blockIdx.y == 0) ww += we * we;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (j < ni) {
+        const double x = V[i0 + j][e];
+        aa[j] += x * ve;
+        at[j] += x * we;
+      }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < 17; j++) {
+    double v = j < 8 ? aa[j] : (j < 16 ? at[j - 8] : ww);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[j][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 17) {
+    const int j = threadIdx.x;
+    double t = 0.0;
+    for (int q = 0; q < kBS / 64; q++) t += red[j][q];
+    long slot = -1;
+    if (j < 8 && j < ni) slot = i0 + j;
+    else if (j >= 8 && j < 16 && j - 8 < ni) slot = k + 1 + i0 + (j - 8);
+    else if (j == 16 && blockIdx.y == 0) slot = 2 * k + 2;
+    if (slot >= 0) part[slot * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// The update pass of the same step (coefficients c: a_0..a_k-1, t_0..t_k-1,
+// then t_k', 1/rho, 1/eta): V_k <- (V_k - sum a_i V_i) / rho and
+// V_k+1 = (w - sum t_i V_i - t_k' V_k) / eta, reading V_0..V_k-1 once.
+__global__ void k_lsgs_update(long n, int k, double *const *__restrict__ V, const double *__restrict__ c,
+                              const double *__restrict__ w) {
+  const double tk = c[2 * k], irho = c[2 * k + 1], ieta = c[2 * k + 2];
+  double *vk = V[k], *vn = V[k + 1];
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < k; i++) {
+      const double x = V[i][e];
+      s1 += c[i] * x;
+      s2 += c[k + i] * x;
+    }
+    const double v = (vk[e] - s1) * irho;
+    vk[e] = v;
+    vn[e] = (w[e] - s2 - tk * v) * ieta;
   }
 }
 
@@ -847,6 +911,7 @@ struct ptmf_mg_s {
   double *coef_dev = nullptr, *coef_host = nullptr;  // Krylov coefficients (pinned upload)
   // dense coarse factor
   rocblas_handle blas = nullptr;
+  ptmf::MFFactor *mf = nullptr; // multifrontal Cholesky of level 0 (sparse coarse factor)
   double *Adense = nullptr;     // explicit inverse, full symmetric (column-major)
   double *dense_part = nullptr; // partial sums of the inverse's product
   int ndense = 0;
@@ -1081,7 +1146,6 @@ int vcycle(ptmf_mg_s *mg, int l, int lc, const T *b, T *x) {
   PTMF_TRY(matmult<T>(L, x, r));
   k_residual<T><<<nblocks(L.n), kBS, 0, s>>>(L.n, b, r, r);
   PTMF_HIP(hipGetLastError());
-  const Box &bf = L.op->box, &bcx = C.op->box;
   PTMF_TRY(restrict_<T>(mg, l, r, bc));
   // coarse correction vanishing on the Dirichlet dofs: the coarse Dirichlet
   // rows are identities, so a nonzero restricted residual there would act as
@@ -1101,7 +1165,7 @@ int vcycle64(ptmf_mg_s *mg, int l, int lc, const double *b, double *x) {
 // Coarse solve on level lc: dense Cholesky on level 0, otherwise FP64 PCG
 // preconditioned by an FP64 V-cycle over levels lc..0.
 int coarse_solve(ptmf_mg_s *mg, int lc, const double *b, double *x) {
-  if (lc == 0) return dense_solve(mg, b, x);
+  if (lc == 0) return mg->mf ? mf_solve(mg->mf, b, x, mg->stream) : dense_solve(mg, b, x);
   MGLevel &L = mg->lev[lc];
   hipStream_t s = mg->stream;
   const long n = L.n;
@@ -1238,7 +1302,7 @@ int place_dirichlet(ptmf_mg_s *mg, MGLevel &L) {
 int assemble_galerkin(ptmf_mg_s *mg, int l) {
   MGLevel &Cl = mg->lev[l], &Fl = mg->lev[l + 1];
   hipStream_t s = mg->stream;
-  const Box &bc = Cl.op->box, &bf = Fl.op->box;
+  const Box &bc = Cl.op->box;
   const long N = bc.nnodes;
   if (!Cl.S64) PTMF_TRY(dalloc((void **)&Cl.S64, sizeof(double) * 1125 * N));
   if (!Cl.S32) PTMF_TRY(dalloc((void **)&Cl.S32, sizeof(float) * 1125 * N));
@@ -1269,8 +1333,9 @@ int assemble_galerkin(ptmf_mg_s *mg, int l) {
 int assemble_dense(ptmf_mg_s *mg) {
   MGLevel &L = mg->lev[0];
   const long n = L.n;
-  PTMF_CHECK(n <= 32768, PTMF_ERR_UNSUPPORTED,
-             "coarsest level has %ld dofs; the dense coarse solve takes at most 32768", n);
+  PTMF_CHECK(n <= kDenseMaxDofs, PTMF_ERR_UNSUPPORTED,
+             "coarsest level has %ld dofs; the dense coarse solve takes at most %ld (coarse_factor 2: "
+             "sparse multifrontal Cholesky)", n, kDenseMaxDofs);
   mg->ndense = (int)n;
   if (!mg->Adense) PTMF_TRY(dalloc((void **)&mg->Adense, sizeof(double) * n * n));
   hipStream_t s = mg->stream;
@@ -1312,6 +1377,51 @@ int assemble_dense(ptmf_mg_s *mg) {
   return PTMF_OK;
 }
 
+// Level l's operator as its 125-point stencil (the layout of the Galerkin
+// levels), probed through the level's own MatMult with the 375 colourings;
+// Dirichlet dofs give unit rows and zero columns, as in the MatMult.
+int probe_stencil(ptmf_mg_s *mg, int l, double *S) {
+  MGLevel &L = mg->lev[l];
+  hipStream_t s = mg->stream;
+  const Box &b = L.op->box;
+  const long N = b.nnodes;
+  PTMF_HIP(hipMemsetAsync(S, 0, sizeof(double) * 1125 * N, s));
+  double *v = L.p64[0], *w = L.r64;
+  for (int ck = 0; ck < 5; ck++)
+    for (int cj = 0; cj < 5; cj++)
+      for (int ci = 0; ci < 5; ci++)
+        for (int d = 0; d < 3; d++) {
+          k_probe<<<nblocks(N), kBS, 0, s>>>(b.X, b.Y, b.Z, ci, cj, ck, d, v);
+          PTMF_TRY(matmult<double>(L, v, w));
+          k_probe_scatter<<<nblocks(N), kBS, 0, s>>>(b.X, b.Y, b.Z, ci, cj, ck, d, w, S);
+          PTMF_HIP(hipGetLastError());
+        }
+  return PTMF_OK;
+}
+
+// Exact factor of level 0: the dense explicit inverse (at most kDenseMaxDofs)
+// or the multifrontal Cholesky of its assembled stencil (any size).
+int setup_coarse(ptmf_mg_s *mg) {
+  MGLevel &L = mg->lev[0];
+  mf_destroy(mg->mf);
+  mg->mf = nullptr;
+  const int cf = mg->opt.coarse_factor;
+  if (cf == 1 || (cf == 0 && L.n <= kDenseMaxDofs)) return assemble_dense(mg);
+  // the dense inverse of an earlier set-up must not be used any more
+  (void)hipFree(mg->Adense);
+  (void)hipFree(mg->dense_part);
+  mg->Adense = mg->dense_part = nullptr;
+  mg->ndense = 0;
+  if (L.galerkin) return mf_factor(L.op->box, L.S64, mg->stream, &mg->mf);
+  double *S = nullptr;
+  PTMF_TRY(dalloc((void **)&S, sizeof(double) * 1125 * L.op->box.nnodes));
+  int st = probe_stencil(mg, 0, S);
+  if (st == PTMF_OK) st = mf_factor(L.op->box, S, mg->stream, &mg->mf);
+  (void)hipStreamSynchronize(mg->stream);
+  (void)hipFree(S);
+  return st;
+}
+
 void drop_graph(ptmf_mg_s *mg) {
   if (mg->vexec) (void)hipGraphExecDestroy(mg->vexec);
   if (mg->vgraph) (void)hipGraphDestroy(mg->vgraph);
@@ -1320,12 +1430,39 @@ void drop_graph(ptmf_mg_s *mg) {
   mg->vcalls = 0;
 }
 
-// Right-preconditioned flexible GMRES(restart) on vectors of length n with
-// classical Gram-Schmidt applied twice (fused multi-vector dot and update
-// kernels, coefficients on the host).  apply(v, y): y = K v; prec(r, z):
+// One reduction of the low-synchronisation Gram-Schmidt step for column k:
+// out[i] = <V_i, V_k> (i <= k), out[k + 1 + i] = <V_i, w> (i <= k),
+// out[2k + 2] = <w, w>; one pass over V_0..V_k and w, one host synchronisation.
+int lsgs_reduce(ptmf_mg_s *mg, long n, int k, const double *const *dV, const double *w, double *out) {
+  PTMF_CHECK(k >= 0 && 2 * k + 3 <= kRedSlots, PTMF_ERR_ARG, "lsgs_reduce: column %d", k);
+  const int nch = (k + 1 + 7) / 8;
+  k_lsgs_partial<<<dim3(kRedBlocks, (unsigned)nch), kBS, 0, mg->stream>>>(n, k, dV, w, mg->red_part);
+  k_mdot_final<<<1, kBS, 0, mg->stream>>>(2 * k + 3, kRedBlocks, mg->red_part, mg->red_out);
+  PTMF_HIP(hipGetLastError());
+  PTMF_HIP(hipMemcpyAsync(mg->red_host, mg->red_out, sizeof(double) * (2 * k + 3), hipMemcpyDeviceToHost,
+                          mg->stream));
+  PTMF_HIP(hipStreamSynchronize(mg->stream));
+  memcpy(out, mg->red_host, sizeof(double) * (2 * k + 3));
+  return PTMF_OK;
+}
+
+// Right-preconditioned flexible GMRES(restart) on vectors of length n with a
+// low-synchronisation Gram-Schmidt process (one reduction and one update
+// pass over the basis per iteration; the re-orthogonalisation of each basis
+// vector is delayed to the next iteration's reduction, the DCGS2 scheme of
+// Swirydowicz, Langou, Ananthan, Yang and Thomas, "Low synchronization
+// Gram-Schmidt and generalized minimal residual algorithms", Numer. Linear
+// Algebra Appl. 28 (2021) e2343).  At iteration k the one reduction gives
+// <V_i, V_k>, <V_i, w> and <w, w> for w = A z_k; V_k (orthogonalised once,
+// normalised by an estimate) is then orthogonalised again and normalised
+// exactly, column k-1 of the Hessenberg matrix is corrected for that change
+// and only then reduced by its Givens rotation, and w is orthogonalised
+// against the corrected basis in the same update pass that writes V_k+1.
+// So the residual estimate lags one iteration: at convergence one
+// preconditioner application is not used.  apply(v, y): y = K v; prec(r, z):
 // z ~ K^-1 r (may vary between calls).  The true residual is recomputed at
-// every restart and at the end; stops when it (or the Arnoldi estimate) drops
-// below rtol * ||b||.
+// every restart and at the end; stops when it (or the Arnoldi estimate)
+// drops below rtol * ||b||.
 template <typename Apply, typename Prec>
 int fgmres_core(ptmf_mg_s *mg, KrylovSpace &ks, long n, Apply apply, Prec prec, const double *b, double *x,
                 double rtol, int maxit, int restart, int *its_out, double *rnorm_out, double *history,
@@ -1348,18 +1485,38 @@ int fgmres_core(ptmf_mg_s *mg, KrylovSpace &ks, long n, Apply apply, Prec prec, 
   }
   double *w = ks.w;
   std::vector<double> H((restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart);
+  std::vector<double> red(2 * restart + 3);
   double bnorm2;
   PTMF_TRY(dot(mg, n, b, b, &bnorm2));
   const double bnorm = sqrt(bnorm2);
-  int its = 0, hl = 0;
+  int its = 0, started = 0, hl = 0;
   double rnorm = 0.0;
   if (bnorm == 0.0) {
-    PTMF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    PTMF_TRY(launch_fill<double>(x, 0.0, n, s));
     PTMF_HIP(hipStreamSynchronize(s));
     if (its_out) *its_out = 0;
     if (rnorm_out) *rnorm_out = 0.0;
     return PTMF_OK;
   }
+  // Givens reduction of the (final) column j of H; returns |g_j+1|
+  auto reduce_column = [&](int j) {
+    double h[kMaxDots + 1];
+    for (int i = 0; i <= j + 1; i++) h[i] = H[i * restart + j];
+    for (int i = 0; i < j; i++) {
+      const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+      h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+      h[i] = t;
+    }
+    const double den = sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+    cs[j] = den > 0 ? h[j] / den : 1.0;
+    sn[j] = den > 0 ? h[j + 1] / den : 0.0;
+    h[j] = den;
+    h[j + 1] = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    for (int i = 0; i <= j; i++) H[i * restart + j] = h[i];
+    return fabs(g[j + 1]);
+  };
   while (true) {
     PTMF_TRY(apply(x, w));
     k_residual<double><<<nblocks(n), kBS, 0, s>>>(n, b, w, ks.V[0]);
@@ -1369,53 +1526,86 @@ int fgmres_core(ptmf_mg_s *mg, KrylovSpace &ks, long n, Apply apply, Prec prec, 
     rnorm = sqrt(rr);
     PTMF_CHECK(isfinite(rnorm), PTMF_ERR_DIVERGED, "fgmres: non-finite residual");
     if (history && hl < history_len) history[hl++] = rnorm;
-    if (rnorm <= rtol * bnorm || its >= maxit) break;
+    if (rnorm <= rtol * bnorm || started >= maxit) break;
     PTMF_TRY(scale<double>(s, n, 1.0 / rnorm, ks.V[0], ks.V[0]));
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = rnorm;
-    int j = 0;
-    while (j < restart && its < maxit) {
-      PTMF_TRY(prec(ks.V[j], ks.Z[j]));
-      PTMF_TRY(apply(ks.Z[j], w));
-      double h[kMaxDots], h2[kMaxDots];
-      PTMF_TRY(mdot(mg, n, j + 1, ks.dV, w, h));
-      k_maxpy<<<nblocks(n), kBS, 0, s>>>(n, j + 1, ks.dV, mg->red_out, w);
-      PTMF_HIP(hipGetLastError());
-      PTMF_TRY(mdot(mg, n, j + 1, ks.dV, w, h2));
-      k_maxpy<<<nblocks(n), kBS, 0, s>>>(n, j + 1, ks.dV, mg->red_out, w);
-      PTMF_HIP(hipGetLastError());
-      for (int i = 0; i <= j; i++) h[i] += h2[i];
-      double ww;
-      PTMF_TRY(dot(mg, n, w, w, &ww));
-      const double hn = sqrt(ww);
-      PTMF_CHECK(isfinite(hn), PTMF_ERR_DIVERGED, "fgmres: non-finite Arnoldi vector");
-      for (int i = 0; i < j; i++) {
-        const double t = cs[i] * h[i] + sn[i] * h[i + 1];
-        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-        h[i] = t;
+    int ncol = 0;            // final columns of this cycle
+    double eta_prev = 0.0;   // provisional h_k,k-1: the estimate V_k was normalised with
+    for (int k = 0;; k++) {
+      // extend by column k, or (end of cycle) only finalise column k-1
+      const bool extend = k < restart && started < maxit;
+      if (!extend && k == 0) break;
+      const double *wv = w;
+      if (extend) {
+        PTMF_TRY(prec(ks.V[k], ks.Z[k]));
+        PTMF_TRY(apply(ks.Z[k], w));
+        started++;
+      } else {
+        wv = ks.V[k];
       }
-      const double den = sqrt(h[j] * h[j] + hn * hn);
-      cs[j] = den > 0 ? h[j] / den : 1.0;
-      sn[j] = den > 0 ? hn / den : 0.0;
-      h[j] = den;
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
-      for (int i = 0; i <= j; i++) H[i * restart + j] = h[i];
-      its++;
-      j++;
-      if (history && hl < history_len) history[hl++] = fabs(g[j]);
-      if (fabs(g[j]) <= rtol * bnorm || hn == 0.0) break;
-      PTMF_TRY(scale<double>(s, n, 1.0 / hn, w, ks.V[j]));
+      PTMF_TRY(lsgs_reduce(mg, n, k, ks.dV, wv, red.data()));
+      const double *a = red.data(), *t = red.data() + (k + 1);
+      const double ww = red[2 * k + 2];
+      PTMF_CHECK(isfinite(ww) && isfinite(a[k]), PTMF_ERR_DIVERGED, "fgmres: non-finite Arnoldi vector");
+      double rho = 1.0;
+      if (k > 0) {
+        // V_k = rho V_k' + sum_i<k a_i V_i: column k-1 in the corrected basis
+        double aa = 0.0;
+        for (int i = 0; i < k; i++) aa += a[i] * a[i];
+        const double r2 = a[k] - aa;
+        rho = r2 > 1e-30 * a[k] ? sqrt(r2) : 0.0;  // 0: A z_k-1 lies in the basis (breakdown)
+        for (int i = 0; i < k; i++) H[i * restart + k - 1] += eta_prev * a[i];
+        H[k * restart + k - 1] = eta_prev * rho;
+        const double est = reduce_column(k - 1);
+        its++;
+        ncol = k;
+        if (history && hl < history_len) history[hl++] = est;
+        if (est <= rtol * bnorm || rho == 0.0) break;
+      }
+      if (!extend) break;
+      // column k against the corrected basis: t_k' = <V_k', w>
+      double tk = t[k];
+      for (int i = 0; i < k; i++) tk -= a[i] * t[i];
+      tk = rho > 0.0 ? tk / rho : 0.0;
+      double e2 = ww - tk * tk;
+      for (int i = 0; i < k; i++) {
+        H[i * restart + k] = t[i];
+        e2 -= t[i] * t[i];
+      }
+      H[k * restart + k] = tk;
+      // the norm of the orthogonalised w by Pythagoras; only a scale: the
+      // next reduction measures V_k+1 exactly and corrects column k
+      double eta = e2 > 1e-14 * ww ? sqrt(e2) : (ww > 0.0 ? 1e-7 * sqrt(ww) : 1.0);
+      double *c = mg->coef_host;
+      for (int i = 0; i < k; i++) {
+        c[i] = a[i];
+        c[k + i] = t[i];
+      }
+      c[2 * k] = tk;
+      c[2 * k + 1] = k > 0 ? 1.0 / rho : 1.0;
+      c[2 * k + 2] = 1.0 / eta;
+      PTMF_HIP(hipMemcpyAsync(mg->coef_dev, c, sizeof(double) * (2 * k + 3), hipMemcpyHostToDevice, s));
+      k_lsgs_update<<<nblocks(n), kBS, 0, s>>>(n, k, (double *const *)ks.dV, mg->coef_dev, w);
+      PTMF_HIP(hipGetLastError());
+      // the coefficient buffer is reused by the next iteration
+      PTMF_HIP(hipStreamSynchronize(s));
+      eta_prev = eta;
     }
-    for (int i = j - 1; i >= 0; i--) {
-      double t = g[i];
-      for (int k = i + 1; k < j; k++) t -= H[i * restart + k] * y[k];
-      y[i] = t / H[i * restart + i];
+    // x += Z y over the final columns
+    const int j = ncol;
+    if (j > 0) {
+      for (int i = j - 1; i >= 0; i--) {
+        double tt = g[i];
+        for (int k = i + 1; k < j; k++) tt -= H[i * restart + k] * y[k];
+        y[i] = tt / H[i * restart + i];
+      }
+      memcpy(mg->coef_host, y.data(), sizeof(double) * j);
+      PTMF_HIP(hipMemcpyAsync(mg->coef_dev, mg->coef_host, sizeof(double) * j, hipMemcpyHostToDevice, s));
+      k_mupdate<<<nblocks(n), kBS, 0, s>>>(n, j, ks.dZ, mg->coef_dev, x);
+      PTMF_HIP(hipGetLastError());
+      PTMF_HIP(hipStreamSynchronize(s));
     }
-    memcpy(mg->coef_host, y.data(), sizeof(double) * j);
-    PTMF_HIP(hipMemcpyAsync(mg->coef_dev, mg->coef_host, sizeof(double) * j, hipMemcpyHostToDevice, s));
-    k_mupdate<<<nblocks(n), kBS, 0, s>>>(n, j, ks.dZ, mg->coef_dev, x);
-    PTMF_HIP(hipGetLastError());
   }
   PTMF_HIP(hipStreamSynchronize(s));
   if (its_out) *its_out = its;
@@ -1491,6 +1681,7 @@ int ptmf_mg_default_options(ptmf_mg_options *o) {
   o->transfer = 0;
   o->smoother_diag = 0;
   o->use_graph = 1;
+  o->coarse_factor = 0;
   return PTMF_OK;
 }
 
@@ -1527,13 +1718,13 @@ int ptmf_mg_create(int mx, int my, int mz, int nlevels, ptmf_mg *out) {
     for (void **p : a4)
       if (st == PTMF_OK) st = ptmf::dalloc(p, b4);
   }
-  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->red_part, sizeof(double) * ptmf::kRedBlocks * ptmf::kMaxDots);
-  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->red_out, sizeof(double) * ptmf::kMaxDots);
-  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->coef_dev, sizeof(double) * ptmf::kMaxDots);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->red_part, sizeof(double) * ptmf::kRedBlocks * ptmf::kRedSlots);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->red_out, sizeof(double) * ptmf::kRedSlots);
+  if (st == PTMF_OK) st = ptmf::dalloc((void **)&mg->coef_dev, sizeof(double) * ptmf::kRedSlots);
   if (st == PTMF_OK) {
-    he = hipHostMalloc((void **)&mg->red_host, sizeof(double) * ptmf::kMaxDots, hipHostMallocDefault);
+    he = hipHostMalloc((void **)&mg->red_host, sizeof(double) * ptmf::kRedSlots, hipHostMallocDefault);
     if (he == hipSuccess)
-      he = hipHostMalloc((void **)&mg->coef_host, sizeof(double) * ptmf::kMaxDots, hipHostMallocDefault);
+      he = hipHostMalloc((void **)&mg->coef_host, sizeof(double) * ptmf::kRedSlots, hipHostMallocDefault);
     if (he != hipSuccess) st = ptmf_set_error(PTMF_ERR_NOMEM, "mg_create: pinned buffer: %s", hipGetErrorString(he));
   }
   if (st == PTMF_OK) {
@@ -1579,6 +1770,7 @@ int ptmf_mg_destroy(ptmf_mg *pmg) {
   (void)hipFree(mg->red_out);
   (void)hipFree(mg->coef_dev);
   if (mg->coef_host) (void)hipHostFree(mg->coef_host);
+  ptmf::mf_destroy(mg->mf);
   (void)hipFree(mg->Adense);
   (void)hipFree(mg->dense_part);
   (void)hipFree(mg->dinfo);
@@ -1752,7 +1944,9 @@ int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt) {
       PTMF_TRY(ptmf::place_dirichlet(mg, L));
     }
   }
-  PTMF_TRY(ptmf::assemble_dense(mg));
+  PTMF_CHECK(o.coarse_factor >= 0 && o.coarse_factor <= 2, PTMF_ERR_UNSUPPORTED,
+             "mg_setup: coarse_factor %d (0 auto, 1 dense, 2 sparse)", o.coarse_factor);
+  PTMF_TRY(ptmf::setup_coarse(mg));
   // coarse PCG workspace (level levels_mg coarse index)
   const int lc = mg->L - o.levels_mg;
   if (lc > 0) {
@@ -1764,6 +1958,17 @@ int ptmf_mg_setup(ptmf_mg mg, const ptmf_mg_options *opt) {
   }
   PTMF_HIP(hipStreamSynchronize(s));
   mg->is_setup = true;
+  return PTMF_OK;
+}
+
+int ptmf_mg_coarse_info(ptmf_mg mg, int *kind, long *bytes, int *depths, double *setup_ms) {
+  PTMF_CHECK(mg, PTMF_ERR_ARG, "mg_coarse_info: null");
+  PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_coarse_info: not set up");
+  const bool sparse = mg->mf != nullptr;
+  if (kind) *kind = sparse ? 2 : 1;
+  if (bytes) *bytes = sparse ? 8 * ptmf::mf_factor_doubles(mg->mf) : 8L * mg->ndense * mg->ndense;
+  if (depths) *depths = sparse ? ptmf::mf_depths(mg->mf) : 0;
+  if (setup_ms) *setup_ms = sparse ? ptmf::mf_setup_ms(mg->mf) : 0.0;
   return PTMF_OK;
 }
 
@@ -1796,8 +2001,8 @@ int vcycle_body(ptmf_mg_s *mg, const double *b, double *x) {
   return cast<float, double>(mg->stream, F.n, F.x32, x);
 }
 
-ptmf_mg_s::GraphKey graph_key(const ptmf_op_s *op) {
-  ptmf_mg_s::GraphKey k;
+void graph_key(const ptmf_op_s *op, ptmf_mg_s::GraphKey *out) {
+  ptmf_mg_s::GraphKey &k = *out;
   k.state = op->state;
   k.TX = op->tiling.TX;
   k.TY = op->tiling.TY;
@@ -1808,7 +2013,6 @@ ptmf_mg_s::GraphKey graph_key(const ptmf_op_s *op) {
   k.et64 = op->et64;
   k.et32 = op->et32;
   k.nbc = op->nbc;
-  return k;
 }
 
 // Capture one V-cycle from gb into gx as a HIP graph.  Everything the cycle
@@ -1832,13 +2036,15 @@ int capture_vcycle(ptmf_mg_s *mg) {
   }
   mg->vgraph = g;
   mg->vexec = ex;
-  for (int l = 0; l < mg->L; l++) mg->gkey[l] = graph_key(mg->lev[l].op);
+  for (int l = 0; l < mg->L; l++) graph_key(mg->lev[l].op, &mg->gkey[l]);
   return PTMF_OK;
 }
 
 bool graph_stale(const ptmf_mg_s *mg) {
   for (int l = 0; l < mg->L; l++) {
-    const ptmf_mg_s::GraphKey k = graph_key(mg->lev[l].op), &c = mg->gkey[l];
+    ptmf_mg_s::GraphKey k;
+    graph_key(mg->lev[l].op, &k);
+    const ptmf_mg_s::GraphKey &c = mg->gkey[l];
     if (k.state != c.state || k.TX != c.TX || k.TY != c.TY || k.ZC != c.ZC || k.variant != c.variant ||
         k.faces != c.faces || k.bcidx != c.bcidx || k.et64 != c.et64 || k.et32 != c.et32 || k.nbc != c.nbc)
       return true;
@@ -2268,18 +2474,6 @@ __global__ void kb_prolong_add(const int *__restrict__ act, const long *__restri
   }
 }
 
-// mirror the lower triangle of each box's inverse into its upper triangle
-// (column-major), so the product below reads whole columns
-__global__ void kb_symmetrize(const long *__restrict__ n, double *const *A) {
-  const int b = blockIdx.y;
-  const long nb = n[b];
-  double *Ab = A[b];
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nb * nb; t += (long)gridDim.x * blockDim.x) {
-    const long i = t % nb, j = t / nb;  // entry (i, j), column j
-    if (i < j) Ab[i + j * nb] = Ab[j + i * nb];
-  }
-}
-
 // y = A x per box with the full (symmetrised) inverse: consecutive threads
 // read consecutive rows of a column
 __global__ void kb_symv(const int *__restrict__ act, const long *__restrict__ n, const double *const *A, const double *const *x, double *const *y) {
@@ -2640,6 +2834,12 @@ int ptmf_bmg_create(const ptmf_mg *mgs, int nbox, ptmf_bmg *out) {
   std::vector<const double *> A(nbox);
   std::vector<long> n0(nbox);
   for (int b = 0; b < nbox; b++) {
+    if (!m->mgs[b]->Adense) {
+      st = ptmf_set_error(PTMF_ERR_UNSUPPORTED,
+                          "bmg_create: box %d has no dense coarse inverse (the batched V-cycle needs "
+                          "coarse_factor 0 or 1 and at most 32768 coarsest dofs)", b);
+      break;
+    }
     A[b] = m->mgs[b]->Adense;
     n0[b] = m->mgs[b]->ndense;
     if (st == PTMF_OK && (!A[b] || n0[b] != m->mgs[b]->lev[0].n))
@@ -2647,13 +2847,7 @@ int ptmf_bmg_create(const ptmf_mg *mgs, int nbox, ptmf_bmg *out) {
   }
   up((void **)&m->dA, A.data(), sizeof(double *) * nbox);
   up((void **)&m->dn0, n0.data(), sizeof(long) * nbox);
-  if (st == PTMF_OK) {
-    // the boxes' own dense solves read the lower triangle only; the upper one
-    // is free to hold the mirror image
-    ptmf::kb_symmetrize<<<dim3(256, (unsigned)nbox), ptmf::kBS, 0, m->stream>>>(m->dn0, (double *const *)m->dA);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess)
-      st = ptmf_set_error(PTMF_ERR_DEVICE, "bmg_create: symmetrise");
-  }
+  // (each box's dense inverse is already stored full, mirrored at its set-up)
   // flexible-CG workspace
   const ptmf::BLevel &F = m->lev[m->L - 1];
   (void)F;

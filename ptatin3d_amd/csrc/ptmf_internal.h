@@ -82,6 +82,7 @@ struct ptmf_op_s {
   long nbc = 0;
   int has_coords = 0, has_eta = 0, has_bc = 0;
   unsigned long state = 0;        // bumped by every coefficient change
+  int device = 0;                 // device the operator was created on
   hipStream_t stream = nullptr;   // stream all work of this operator runs on
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ptmf::Tiling tiling;
@@ -165,5 +166,16 @@ int launch_a11_l1(const ptmf_op_s *op, double *d);
 int launch_schur_blocks(const ptmf_op_s *op, double *Me, double *Minv);
 int launch_schur_apply(const ptmf_op_s *op, const double *Minv, const double *rp, double *yp);
 int launch_a12(const ptmf_op_s *op, const double *p, double *yu);
+// Multifrontal Cholesky of a level operator given as its 125-point 3x3-block
+// stencil S (device, S[((o*9)+3r+c)*N + m] = A(3m+r, 3(m+off_o)+c), SPD with
+// unit Dirichlet rows), nested-dissection ordered (coarse.hip).  mf_solve is
+// kernels only (capturable, no allocation); b and x are device FP64 arrays.
+struct MFFactor;
+int mf_factor(const Box &box, const double *S, hipStream_t stream, MFFactor **out);
+int mf_solve(const MFFactor *m, const double *b, double *x, hipStream_t s);
+void mf_destroy(MFFactor *m);
+long mf_factor_doubles(const MFFactor *m);
+double mf_setup_ms(const MFFactor *m);
+int mf_depths(const MFFactor *m);
 int launch_a21(const ptmf_op_s *op, const double *u, double *yp);
 }  // namespace ptmf

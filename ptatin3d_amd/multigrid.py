@@ -85,6 +85,14 @@ class GMG:
     def setup(self, opt: MGOptions | None = None) -> None:
         call("ptmf_mg_setup", self.handle, C.byref(opt) if opt is not None else None)
 
+    def coarse_info(self) -> dict:
+        """The coarse factor of the last set-up (dense inverse or sparse
+        multifrontal Cholesky), its bytes, tree depth and factorisation ms."""
+        k, b, d, ms = C.c_int(), C.c_long(), C.c_int(), C.c_double()
+        call("ptmf_mg_coarse_info", self.handle, C.byref(k), C.byref(b), C.byref(d), C.byref(ms))
+        return {"kind": {1: "dense", 2: "sparse"}[k.value], "bytes": b.value, "depths": d.value,
+                "setup_ms": ms.value}
+
     def eigs(self, level: int) -> tuple[float, float]:
         a, b = C.c_double(), C.c_double()
         call("ptmf_mg_level_eigs", self.handle, int(level), C.byref(a), C.byref(b))
