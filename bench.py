@@ -75,7 +75,12 @@ def main() -> int:
     ap.add_argument("--tiling", type=int, nargs=3, metavar=("TX", "TY", "ZC"),
                     help="fixed MatMult tiling (no autotuning)")
     ap.add_argument("--no-solve", action="store_true", help="skip the mixed-precision GMG solve leg")
-    ap.add_argument("--solve-m", type=int, default=64)
+    ap.add_argument("--solve-m", type=int, default=128, help="elements per direction of the solve leg")
+    ap.add_argument("--solve-levels", type=int, default=4)
+    ap.add_argument("--solve-cheb", type=int, default=3, help="Chebyshev steps per smoothing")
+    ap.add_argument("--solve-krylov", choices=("fgmres", "fcg"), default="fgmres")
+    ap.add_argument("--solve-restart", type=int, default=30)
+    ap.add_argument("--solve-galerkin", type=int, default=0, help="Galerkin coarse levels of the solve leg")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--batch", type=int, default=256, help="boxes in the batched-MatMult leg (0: skip)")
     ap.add_argument("--batch-solve", type=int, default=0,
@@ -241,7 +246,8 @@ def main() -> int:
     if args.batch > 0:
         out["batch"] = batch_leg(args.batch)
     if not args.no_solve:
-        out["solve"] = solve_leg(args.solve_m)
+        out["solve"] = solve_leg(args.solve_m, args.solve_levels, args.solve_cheb, args.solve_krylov,
+                                 args.solve_restart, args.solve_galerkin)
     if args.batch_solve > 0:
         out["batch_solve"] = batch_solve_leg(args.batch_solve)
     if args.stokes:
@@ -424,40 +430,46 @@ def batch_solve_leg(nbox: int) -> dict:
     return res
 
 
-def solve_leg(m: int) -> dict:
+def solve_leg(m: int, levels: int = 4, cheb: int = 3, krylov: str = "fgmres", restart: int = 30,
+              galerkin: int = 0, maxit: int = 4000) -> dict:
     """Layered viscosity 1e3/1e1/1e-3 with 64 weak ellipsoids (contrast 1e6),
-    smooth body force, all-face Dirichlet.  Flexible CG (FP64) preconditioned
-    by a geometric V-cycle down to 4^3 (dense Cholesky there): Chebyshev(10)
-    on [0.02, 1.1] x lambda_max of D^-1 A with point Jacobi, quadrature-point
-    local viscosity means on the rediscretised levels, Galerkin operators on
-    the levels of at most 8^3 elements.  The V-cycle runs in FP32 or in FP64;
-    both are solved to ||b - A x|| <= 1e-8 ||b|| (FP64 residual of the FP64
-    operator), and each solution is compared with a 1e-11 FP64 solve."""
+    smooth body force, all-face Dirichlet, on m^3 Q2 elements.  The A11 solve
+    of the configured workload: FGMRES(restart) in FP64 (or flexible CG)
+    preconditioned by a `levels`-level geometric V-cycle with
+    Chebyshev(cheb)-Jacobi smoothing (esteig 0, 0.2, 0, 1.1 of lambda_max of
+    D^-1 A, as the reference's opts), lattice transfers, quadrature-point
+    local viscosity means on the rediscretised coarse levels, and an exact
+    FP64 coarse solve (multifrontal Cholesky of the assembled coarsest level).
+    The V-cycle (operators, smoother, transfers) runs in FP32 or in FP64;
+    both solves go to ||b - A x|| <= 1e-8 ||b|| (FP64 residual of the FP64
+    operator); each solution is compared with a 1e-11 FP64 solve (the FP64
+    run continued from its own solution)."""
     from ptatin3d_amd import fields as F
     from ptatin3d_amd._lib import DeviceArray, call
     from ptatin3d_amd.multigrid import GMG, default_options, solve_timed
 
     c = F.config3(m)
-    nl = 1
-    while (m >> nl) >= 4:
-        nl += 1
-    gal = sum(1 for l in range(nl) if (m >> (nl - 1 - l)) <= 8)
     res = {"workload": f"{m}^3 Q2 elements, layered 1e3/1e1/1e-3 + 64 weak ellipsoids (contrast 1e6), "
                        "smooth body force, all-face Dirichlet",
-           "velocity_dofs": 3 * F.nnodes(m, m, m), "levels": nl, "galerkin_levels": gal,
-           "krylov": "flexible CG", "smoother": "Chebyshev(10)-Jacobi on [0.02, 1.1] lambda_max"}
+           "velocity_dofs": 3 * F.nnodes(m, m, m), "levels": levels, "galerkin_levels": galerkin,
+           "coarse_level": f"{m >> (levels - 1)}^3",
+           "krylov": f"FGMRES({restart})" if krylov == "fgmres" else "flexible CG",
+           "smoother": f"Chebyshev({cheb})-Jacobi on [0.2, 1.1] lambda_max",
+           "transfers": "DMDA Q1 lattice interpolation, R = P^T",
+           "coarse_solve": "FP64 multifrontal Cholesky of the assembled coarsest level (exact)"}
     runs = {}
     bnorm = float(np.linalg.norm(c["b"]))
-    for tag, prec, rtol in (("fp64", 64, 1e-8), ("fp32", 32, 1e-8), ("tight", 64, 1e-11)):
-        mg = GMG(m, m, m, nl)
+    for tag, prec in (("fp64", 64), ("fp32", 32)):
+        mg = GMG(m, m, m, levels)
         mg.set_coordinates(c["coords"])
-        mg.set_viscosity(nl - 1, c["eta"])
+        mg.set_viscosity(levels - 1, c["eta"])
         mg.coarsen_viscosity("qp-arithmetic")
         mg.set_dirichlet_all_faces()
         t0 = time.perf_counter()
-        mg.setup(default_options(smoother_precision=prec, pre_its=10, post_its=10, galerkin_levels=gal,
-                                 esteig=(0.0, 0.02, 0.0, 1.1)))
+        mg.setup(default_options(smoother_precision=prec, pre_its=cheb, post_its=cheb, galerkin_levels=galerkin,
+                                 coarse_factor=2))
         ts = time.perf_counter() - t0
+        info = mg.coarse_info()
         # the V-cycle alone (the part the smoother precision changes)
         db, dv = DeviceArray.from_host(c["b"]), DeviceArray(c["b"].size)
         mg.vcycle(db, dv)
@@ -469,14 +481,23 @@ def solve_leg(m: int) -> dict:
         vms = 1e3 * (time.perf_counter() - t0) / 5
         db.free()
         dv.free()
-        x, its, rn, hist, secs = solve_timed(mg, c["b"], rtol=rtol, maxit=4000, krylov="fcg")
+        x, its, rn, hist, secs = solve_timed(mg, c["b"], rtol=1e-8, maxit=maxit, restart=restart, krylov=krylov)
         runs[tag] = x
         res[tag] = {"its": its, "seconds": round(secs, 4), "setup_seconds": round(ts, 3),
                     "rel_residual": float(rn / bnorm), "vcycle_ms": round(vms, 3),
-                    "outer_ms_per_it": round(1e3 * secs / max(its, 1) - vms, 3)}
-        mg.destroy()
+                    "outer_ms_per_it": round(1e3 * secs / max(its, 1) - vms, 3),
+                    "coarse_factor_MB": round(info["bytes"] / 1e6, 1),
+                    "coarse_factor_ms": round(info["setup_ms"], 1)}
         print(f"# solve leg {tag}: {its} its, {secs:.2f} s, rel residual {rn / bnorm:.2e}",
               file=sys.stderr, flush=True)
+        if tag == "fp64":
+            xt, its_t, rn_t, _, secs_t = solve_timed(mg, c["b"], rtol=1e-11, maxit=maxit, restart=restart,
+                                                     krylov=krylov, x0=x)
+            runs["tight"] = xt
+            res["tight"] = {"its_after_fp64": its_t, "seconds": round(secs_t, 4), "rel_residual": float(rn_t / bnorm)}
+            print(f"# solve leg tight: {its_t} more its, {secs_t:.2f} s, rel residual {rn_t / bnorm:.2e}",
+                  file=sys.stderr, flush=True)
+        mg.destroy()
     ref = runs["tight"]
     for tag in ("fp64", "fp32"):
         res[tag]["rel_err_vs_tight"] = float(np.linalg.norm(runs[tag] - ref) / np.linalg.norm(ref))

@@ -192,12 +192,15 @@ class GMG:
             pass
 
 
-def solve_timed(mg: GMG, b: np.ndarray, rtol=1e-8, maxit=200, restart=30, krylov="fgmres"):
-    """Upload b, run FGMRES (or flexible CG) from x = 0; returns (x, its,
-    rnorm, history, seconds)."""
+def solve_timed(mg: GMG, b: np.ndarray, rtol=1e-8, maxit=200, restart=30, krylov="fgmres", x0=None):
+    """Upload b, run FGMRES (or flexible CG) from x0 (default 0); returns (x,
+    its, rnorm, history, seconds)."""
     db = DeviceArray.from_host(b)
-    dx = DeviceArray(b.size)
-    dx.zero()
+    if x0 is None:
+        dx = DeviceArray(b.size)
+        dx.zero()
+    else:
+        dx = DeviceArray.from_host(np.asarray(x0, dtype=np.float64))
     call("ptmf_device_synchronize")
     t0 = time.perf_counter()
     if krylov == "fcg":
