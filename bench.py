@@ -118,9 +118,10 @@ def main() -> int:
         A.set_tiling(*args.tiling)
     elif not args.no_tune:
         # the on-the-fly kernels only: they do the reference's per-apply work
-        autotune.tune(A, precision=64, reps=10, max_shapes=4, variants=(0, 1))
+        autotune.tune(A, precision=64, reps=10, max_shapes=4, variants=(0, 1, autotune.COLOUR))
     tiling = A.get_tiling()
     variant = A.get_kernel_variant()
+    wg_tiles = A.get_tiles_per_group()
 
     # warmup, then the timed region: K MatMults bracketed by device syncs
     if args.warmup > 0:
@@ -180,6 +181,7 @@ def main() -> int:
             "parallelism": "single GPU",
             "tiling": list(tiling),
             "kernel_variant": variant,
+            "tiles_per_group": wg_tiles,
         },
         "roofline": {
             "bound": "mfma",
@@ -573,6 +575,14 @@ def stokes_leg(dims, maxit: int, schur: str = "mass", restart: int = 40) -> dict
         db.free()
         dv.free()
         u, p, its, rn, hist, secs = stokes_solve_timed(mg, bu, bp, rtol=1e-8, maxit=maxit, restart=restart)
+        if tag == "fp64":
+            # the FP64 reference: the same FP64 solve continued from its own
+            # solution to a 1e-11 residual
+            ut, pt, its_t, rn_t, _, secs_t = stokes_solve_timed(mg, bu, bp, rtol=1e-11, maxit=maxit, restart=restart,
+                                                                x0=np.concatenate([u, p]))
+            sol["tight"] = (ut, pt)
+            res["tight"] = {"its_after_fp64": its_t, "seconds": round(secs_t, 3), "rel_residual": float(rn_t / bn)}
+            print(f"# stokes leg tight: {its_t} more its, rel residual {rn_t / bn:.2e}", file=sys.stderr, flush=True)
         mg.destroy()
         sol[tag] = (u, p)
         res[tag] = {"its": its, "seconds": round(secs, 3), "ms_per_it": round(1e3 * secs / max(its, 1), 2),
@@ -582,6 +592,11 @@ def stokes_leg(dims, maxit: int, schur: str = "mass", restart: int = 40) -> dict
                     "converged": bool(rn <= 1e-8 * bn)}
         print(f"# stokes leg {tag}: {its} its, {secs:.2f} s, rel residual {rn / bn:.2e}",
               file=sys.stderr, flush=True)
+    ut, pt = sol["tight"]
+    for tag in ("fp32", "fp64"):
+        res[tag]["rel_err_vs_tight"] = {
+            "u": float(np.linalg.norm(sol[tag][0] - ut) / np.linalg.norm(ut)),
+            "p": float(np.linalg.norm(sol[tag][1] - pt) / np.linalg.norm(pt))}
     res["rel_diff_fp32_vs_fp64"] = {
         "u": float(np.linalg.norm(sol["fp32"][0] - sol["fp64"][0]) / np.linalg.norm(sol["fp64"][0])),
         "p": float(np.linalg.norm(sol["fp32"][1] - sol["fp64"][1]) / np.linalg.norm(sol["fp64"][1]))}

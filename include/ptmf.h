@@ -156,9 +156,17 @@ int ptmf_op_get_tiling(ptmf_op op, int *TX, int *TY, int *ZC);
  * loads issued one stage ahead (software pipelined; equal to 0 to rounding);
  * 2 = stored quadrature-point geometry (partial assembly: 9 values per
  * quadrature point, rebuilt after a coefficient or tiling change; same
- * operator to rounding, needs positive 2 w eta / det) */
+ * operator to rounding, needs positive 2 w eta / det); 3 = coloured scatter
+ * (one launch per element parity class, sums added into y directly, no tiles
+ * or face buffers; the colouring of the reference's CUDA backend,
+ * stokes_operators_cuda.cu:337-397, with the 3-lane element kernel) */
 int ptmf_op_set_kernel_variant(ptmf_op op, int variant);
 int ptmf_op_get_kernel_variant(ptmf_op op, int *variant);
+/* Elements per work-group of the variant-0 MatMult: 1, 2 or 4 tiles (one
+ * 64-lane wave and TX*TY elements each) per work-group; the autotuner drives
+ * it.  No reference counterpart (launch shape of the backend). */
+int ptmf_op_set_tiles_per_group(ptmf_op op, int tiles);
+int ptmf_op_get_tiles_per_group(ptmf_op op, int *tiles);
 /* coefficient state counter, bumped by every set_* call (replaces the
  * PetscObjectState stamp of stokes_operators.c:1162-1163) */
 int ptmf_op_state(ptmf_op op, unsigned long *state);

@@ -55,6 +55,8 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_op_set_tiling": (_I, [_P, _I, _I, _I]),
     "ptmf_op_set_kernel_variant": (_I, [_P, _I]),
     "ptmf_op_get_kernel_variant": (_I, [_P, _P]),
+    "ptmf_op_set_tiles_per_group": (_I, [_P, _I]),
+    "ptmf_op_get_tiles_per_group": (_I, [_P, C.POINTER(_I)]),
     "ptmf_op_get_tiling": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "ptmf_a11_spmv": (_I, [_P, _P, _P]),
     "ptmf_a11_spmv_host": (_I, [_P, _P, _P]),

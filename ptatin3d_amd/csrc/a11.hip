@@ -47,6 +47,29 @@ constexpr int kEPW = 21;  // elements per wave (3 lanes each; lane 63 idle)
 constexpr int kSlots = 54;
 constexpr int kWinNodes = 130;  // >= (2TX+1)(2TY+1) for every TX*TY <= 21
 
+// Barrier of one element tile's lanes.  A work-group holds kW waves, each
+// marching through its own tile with its own LDS buffers, so with kW > 1 the
+// waves never wait for each other (their tiles may have different numbers
+// of layers): the barrier is the wave's own (LDS operations of one wave
+// execute in order; the fences keep the compiler from moving them across).
+template <int kWaves>
+__device__ __forceinline__ void tsync() {
+  if constexpr (kWaves == 1) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// lane of the calling thread within its wave
+template <int kWaves>
+__device__ __forceinline__ int wave_lane() {
+  if constexpr (kWaves == 1) return (int)threadIdx.x;
+  else return (int)(threadIdx.x & 63);
+}
+
 template <typename T>
 __device__ __forceinline__ T sel3(int c, double a0, double a1, double a2) {
   return (T)(c == 0 ? a0 : (c == 1 ? a1 : a2));
@@ -168,7 +191,7 @@ __device__ __forceinline__ void interp_plane_t(const T Y[9], T out[9]) {
 // Forward stage: in-plane interpolation of the lane's node plane, exchange
 // through the LDS buffer (slots 0..26 coordinates, 27..53 masked velocity),
 // zeta interpolation to the lane's quadrature plane.
-template <typename T, bool kMask>
+template <typename T, bool kMask, int kWaves = 1>
 __device__ __forceinline__ void plane_forward(int cp, int le, bool writer, const PlaneRaw<T> &r,
                                               T (*sh)[kSlots][kEPW], T Xq[3][9], T Uq[3][9]) {
   T bz[3];
@@ -198,7 +221,7 @@ __device__ __forceinline__ void plane_forward(int cp, int le, bool writer, const
         for (int p = 0; p < 9; p++) sh[cp][27 + 9 * d + p][le] = V[d][p];
     }
   }
-  __syncthreads();
+  tsync<kWaves>();
 #pragma unroll
   for (int d = 0; d < 3; d++)
 #pragma unroll
@@ -232,7 +255,7 @@ struct PCoupling {
   T acc[4];
 };
 
-template <typename T, bool kStokes = false>
+template <typename T, bool kStokes = false, int kWaves = 1>
 __device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq[3][9],
                                          const T Uq[3][9], const T etq9[9],
                                          T (*sh)[kSlots][kEPW], PCoupling<T> *pcp = nullptr) {
@@ -327,21 +350,21 @@ __device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq
       }
       __builtin_amdgcn_wave_barrier();
     }
-  __syncthreads();
+  tsync<kWaves>();
   if (writer) {
 #pragma unroll
     for (int l = 0; l < 3; l++)
 #pragma unroll
       for (int p = 0; p < 9; p++) sh[cp][27 + 9 * l + p][le] = W[l][p];
   }
-  __syncthreads();
+  tsync<kWaves>();
 }
 
 // Backward stage: node plane k = cp gathers the three quadrature planes
 // (zeta), then the transposed in-plane interpolation; out[l][3j+i] is the
 // element result at node (i, j, k = cp), component l.  Ends with a barrier:
 // the exchange buffer is free again on return.
-template <typename T>
+template <typename T, int kWaves = 1>
 __device__ __forceinline__ void plane_backward(int cp, int le, T (*sh)[kSlots][kEPW], T out[3][9]) {
   T bt[3], dt[3];
 #pragma unroll
@@ -359,7 +382,7 @@ __device__ __forceinline__ void plane_backward(int cp, int le, T (*sh)[kSlots][k
               w2 = sh[2][27 + 9 * l + p][le];
       Y[l][p] = bt[0] * w0 + bt[1] * w1 + bt[2] * w2 + dt[0] * z0 + dt[1] * z1 + dt[2] * z2;
     }
-  __syncthreads();
+  tsync<kWaves>();
 #pragma unroll
   for (int l = 0; l < 3; l++) interp_plane_t<T>(Y[l], out[l]);
 }
@@ -368,7 +391,7 @@ __device__ __forceinline__ void plane_backward(int cp, int le, T (*sh)[kSlots][k
 // lane plane cp), result out[l][3j+i] = (A11_e u)(node (i,j,k=cp), comp l).
 // Every lane of the wave must call it (it contains wave-wide barriers); lanes
 // with writer == false (lane 63) never write the exchange buffer.
-template <typename T, bool kMask>
+template <typename T, bool kMask, int kWaves = 1>
 __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer, const T *__restrict__ etap,
                                                     long estride, long base, long X,
                                                     const T *__restrict__ xc,
@@ -379,9 +402,9 @@ __device__ __forceinline__ void element_plane_apply(int cp, int le, bool writer,
   load_plane_nodes<T, kMask>(r, xc, u, bcn, base, X);
   load_plane_eta<T>(r, etap, estride);
   T Xq[3][9], Uq[3][9];
-  plane_forward<T, kMask>(cp, le, writer, r, sh, Xq, Uq);
-  plane_qp<T>(cp, le, writer, Xq, Uq, r.eta, sh);
-  plane_backward<T>(cp, le, sh, out);
+  plane_forward<T, kMask, kWaves>(cp, le, writer, r, sh, Xq, Uq);
+  plane_qp<T, false, kWaves>(cp, le, writer, Xq, Uq, r.eta, sh);
+  plane_backward<T, kWaves>(cp, le, sh, out);
 }
 
 // The same element computation for the saddle-point operator
@@ -480,6 +503,43 @@ __global__ void __launch_bounds__(64)
 }
 
 // ---------------------------------------------------------------------------
+// MatMult with a coloured scatter (kernel variant 3): one launch per element
+// parity class, 21 elements of the class per wave, each lane adding its
+// node plane's sums into y with plain read-add-write (no two elements of a
+// class share a node).  y is cleared first; viscosity read quadrature-point
+// major (eta[q * nel + e]).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(64)
+    a11_colour_kernel(int mx, int my, int mz, long nel, long X, long XY, int color, const T *__restrict__ xc,
+                      const T *__restrict__ eta, const uint8_t *__restrict__ bcn, const T *__restrict__ u,
+                      T *__restrict__ y) {
+  __shared__ T sh[3][kSlots][kEPW];
+  const int lane = threadIdx.x;
+  const int c = lane / kEPW;  // quadrature / node plane; 3 marks the idle lane
+  const int le = lane - c * kEPW;
+  const int cp = c < 3 ? c : 2;
+  bool in_class;
+  const long e = class_element(mx, my, mz, color, (long)blockIdx.x * kEPW + le, in_class);
+  const bool live = (c < 3) && in_class;
+  const long ei = e % mx, er = e / mx, ej = er % my, ek = er / my;
+  const long base = 2 * ei + X * (2 * ej) + XY * (2 * ek + cp);
+  T out[3][9];
+  element_plane_apply<T, true>(cp, le, c < 3, eta + (long)(9 * cp) * nel + e, nel, base, X, xc, bcn, u, sh, out);
+  if (live) {
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          T *q = y + 3 * (base + i + X * j) + l;
+          *q += out[l][3 * j + i];
+        }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // MatMult: tiled z-march with an LDS node-plane window and face buffers
 // ---------------------------------------------------------------------------
 struct TileArgs {
@@ -551,7 +611,7 @@ __global__ void __launch_bounds__(256)
 // Write the completed node plane P of the window: owned nodes to y, the
 // tile's upper-face nodes to the face buffers (x face first, then y, then z).
 // Pure LDS reads and global stores; clears the window entries it reads.
-template <typename T>
+template <typename T, int kWaves = 1>
 __device__ __forceinline__ void flush_plane(const TileArgs &a, long tile, const TileBox &tb, int P,
                                             T *__restrict__ win, T *__restrict__ y,
                                             T *__restrict__ fbx, T *__restrict__ fby,
@@ -562,7 +622,7 @@ __device__ __forceinline__ void flush_plane(const TileArgs &a, long tile, const 
   const int kl = P - 2 * tb.z0;
   const bool oz = (kl < 2 * tb.nzt) || tb.zend;
   const long rowbase = (long)(2 * tb.x0) + a.XY * (long)P;
-  for (int t = threadIdx.x; t < W3 * wY; t += 64) {
+  for (int t = wave_lane<kWaves>(); t < W3 * wY; t += 64) {
     int jl = (int)((float)t * inv);
     if (jl * W3 > t) jl--;
     if ((jl + 1) * W3 <= t) jl++;
@@ -586,7 +646,7 @@ __device__ __forceinline__ void flush_plane(const TileArgs &a, long tile, const 
 
 // One tile's z-march (the work of one wave); shared by the single-box and
 // the batched MatMult kernels.
-template <typename T, bool kMask, bool kStokes = false>
+template <typename T, bool kMask, bool kStokes = false, int kWaves = 1>
 __device__ __forceinline__ void tile_march(const TileArgs &a, long tile, const T *__restrict__ xc,
                                            const T *__restrict__ eta, const uint8_t *__restrict__ bcn,
                                            const T *__restrict__ u, T *__restrict__ y, T *__restrict__ fbx,
@@ -595,7 +655,7 @@ __device__ __forceinline__ void tile_march(const TileArgs &a, long tile, const T
                                            const T *__restrict__ p = nullptr, T *__restrict__ yp = nullptr,
                                            T (*sL)[kEPW] = nullptr) {
   const TileBox tb = tile_box(a, tile);
-  const int lane = threadIdx.x;
+  const int lane = wave_lane<kWaves>();
   const int c = lane / kEPW;
   const int le = lane - c * kEPW;
   const int cp = c < 3 ? c : 2;
@@ -605,7 +665,7 @@ __device__ __forceinline__ void tile_march(const TileArgs &a, long tile, const T
   const int wX = 2 * tb.nxt + 1;
 
   for (int t = lane; t < 9 * kWinNodes; t += 64) (&win[0][0])[t] = T(0);
-  __syncthreads();
+  tsync<kWaves>();
 
   for (int kz = 0; kz < tb.nzt; kz++) {
     const long ek = tb.z0 + kz;
@@ -616,7 +676,7 @@ __device__ __forceinline__ void tile_march(const TileArgs &a, long tile, const T
       element_plane_apply_stokes<T>(cp, le, c < 3, live, eta + tq_index(tile, a.ZC, kz, 9 * cp, le), kEPW, base,
                                     a.X, xc, bcn, u, p + 4 * e, yp + 4 * e, sh, sL, out);
     } else {
-      element_plane_apply<T, kMask>(cp, le, c < 3, eta + tq_index(tile, a.ZC, kz, 9 * cp, le), kEPW, base,
+      element_plane_apply<T, kMask, kWaves>(cp, le, c < 3, eta + tq_index(tile, a.ZC, kz, 9 * cp, le), kEPW, base,
                                     a.X, xc, bcn, u, sh, out);
     }
     // Sum into the window.  For a fixed node slot (i,j) no two lanes touch
@@ -640,12 +700,12 @@ __device__ __forceinline__ void tile_march(const TileArgs &a, long tile, const T
           __builtin_amdgcn_wave_barrier();
         }
     }
-    __syncthreads();
-    flush_plane<T>(a, tile, tb, (int)(2 * ek), win[(2 * kz) % 3], y, fbx, fby, fbz);
-    flush_plane<T>(a, tile, tb, (int)(2 * ek + 1), win[(2 * kz + 1) % 3], y, fbx, fby, fbz);
-    __syncthreads();
+    tsync<kWaves>();
+    flush_plane<T, kWaves>(a, tile, tb, (int)(2 * ek), win[(2 * kz) % 3], y, fbx, fby, fbz);
+    flush_plane<T, kWaves>(a, tile, tb, (int)(2 * ek + 1), win[(2 * kz + 1) % 3], y, fbx, fby, fbz);
+    tsync<kWaves>();
   }
-  flush_plane<T>(a, tile, tb, 2 * (tb.z0 + tb.nzt), win[(2 * tb.nzt) % 3], y, fbx, fby, fbz);
+  flush_plane<T, kWaves>(a, tile, tb, 2 * (tb.z0 + tb.nzt), win[(2 * tb.nzt) % 3], y, fbx, fby, fbz);
 }
 
 template <typename T, bool kMask>
@@ -656,6 +716,23 @@ __global__ void __launch_bounds__(64)
   __shared__ T sh[3][kSlots][kEPW];
   __shared__ T win[3][3 * kWinNodes];
   tile_march<T, kMask>(a, xcd_tile(blockIdx.x, a.ntiles), xc, eta, bcn, u, y, fbx, fby, fbz, sh, win);
+}
+
+// The same z-march with kW tiles per work-group (one wave each, own LDS
+// buffers): work-group g takes tiles kW g .. kW g + kW - 1 of the x-fastest
+// order, work-groups mapped to XCDs as above.  A wave past the last tile
+// returns at once; no barrier spans waves.
+template <typename T, bool kMask, int kWaves>
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+    a11_tile_wg_kernel(TileArgs a, long nwg, const T *__restrict__ xc, const T *__restrict__ eta,
+                       const uint8_t *__restrict__ bcn, const T *__restrict__ u, T *__restrict__ y,
+                       T *__restrict__ fbx, T *__restrict__ fby, T *__restrict__ fbz) {
+  __shared__ T sh[kWaves][3][kSlots][kEPW];
+  __shared__ T win[kWaves][3][3 * kWinNodes];
+  const int w = (int)(threadIdx.x >> 6);
+  const long tile = xcd_tile(blockIdx.x, nwg) * kWaves + w;
+  if (tile >= a.ntiles) return;
+  tile_march<T, kMask, false, kWaves>(a, tile, xc, eta, bcn, u, y, fbx, fby, fbz, sh[w], win[w]);
 }
 
 // Saddle-point MatMult: the A11 z-march with the pressure coupling; y holds
@@ -1242,6 +1319,20 @@ int launch_a11_mult_fb(ptmf_op_s *op, const T *x, T *y, int faces, T *fb) {
   PTMF_CHECK(tl.TX * tl.TY <= kEPW && (2 * tl.TX + 1) * (2 * tl.TY + 1) <= kWinNodes && tl.ZC > 0,
              PTMF_ERR_ARG, "a11: invalid tiling %d x %d x %d", tl.TX, tl.TY, tl.ZC);
   PTMF_CHECK(fb != nullptr, PTMF_ERR_NOT_SETUP, "a11: face buffers not allocated");
+  if (op->variant == 3) {
+    // coloured scatter: no tiles, no face buffers
+    const Box &b = op->box;
+    PTMF_TRY(launch_fill<T>(y, T(0), 3 * b.nnodes, op->stream));
+    for (int color = 0; color < 8; color++) {
+      const long nb = (class_count(b, color) + kEPW - 1) / kEPW;
+      if (nb == 0) continue;
+      a11_colour_kernel<T><<<dim3((unsigned)nb), dim3(64), 0, op->stream>>>(
+          b.mx, b.my, b.mz, b.nel, b.X, b.X * b.Y, color, d.coords, d.eta, op->bcnode, x, y);
+    }
+    PTMF_HIP(hipGetLastError());
+    if (faces == 1) PTMF_TRY(launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
+    return PTMF_OK;
+  }
   const TileArgs a = tile_args(op);
   const long ntiles = tl.ntiles;
   T *fbx = fb, *fby = fbx + 3 * a.fbx_n * ntiles, *fbz = fby + 3 * a.fby_n * ntiles;
@@ -1253,9 +1344,17 @@ int launch_a11_mult_fb(ptmf_op_s *op, const T *x, T *y, int faces, T *fb) {
   } else {
     PTMF_TRY(ensure_eta_tiled<T>(op));
     const T *et = sizeof(T) == 8 ? (const T *)op->et64 : (const T *)op->et32;
+    const int kw = op->wg_tiles;
+    const long nwg = (ntiles + kw - 1) / kw;
     if (op->variant == 1)
       a11_tile_pf_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
           a, d.coords, et, op->bcnode, x, y, fbx, fby, fbz);
+    else if (kw == 2)
+      a11_tile_wg_kernel<T, true, 2><<<dim3((unsigned)nwg), dim3(128), 0, op->stream>>>(
+          a, nwg, d.coords, et, op->bcnode, x, y, fbx, fby, fbz);
+    else if (kw == 4)
+      a11_tile_wg_kernel<T, true, 4><<<dim3((unsigned)nwg), dim3(256), 0, op->stream>>>(
+          a, nwg, d.coords, et, op->bcnode, x, y, fbx, fby, fbz);
     else
       a11_tile_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
           a, d.coords, et, op->bcnode, x, y, fbx, fby, fbz);

@@ -234,9 +234,24 @@ int ptmf_op_set_tiling(ptmf_op op, int TX, int TY, int ZC) {
 
 int ptmf_op_set_kernel_variant(ptmf_op op, int variant) {
   PTMF_CHECK(op, PTMF_ERR_ARG, "set_kernel_variant: null operator");
-  PTMF_CHECK(variant >= 0 && variant <= 2, PTMF_ERR_UNSUPPORTED,
+  PTMF_CHECK(variant >= 0 && variant <= 3, PTMF_ERR_UNSUPPORTED,
              "set_kernel_variant: unknown variant %d", variant);
   op->variant = variant;
+  return PTMF_OK;
+}
+
+int ptmf_op_set_tiles_per_group(ptmf_op op, int tiles) {
+  PTMF_CHECK(op, PTMF_ERR_ARG, "set_tiles_per_group: null operator");
+  PTMF_CHECK(tiles == 1 || tiles == 2 || tiles == 4, PTMF_ERR_UNSUPPORTED,
+             "set_tiles_per_group: %d (1, 2 or 4)", tiles);
+  PTMF_HIP(hipStreamSynchronize(op->stream));
+  op->wg_tiles = tiles;
+  return PTMF_OK;
+}
+
+int ptmf_op_get_tiles_per_group(ptmf_op op, int *tiles) {
+  PTMF_CHECK(op && tiles, PTMF_ERR_ARG, "get_tiles_per_group: null argument");
+  *tiles = op->wg_tiles;
   return PTMF_OK;
 }
 

@@ -70,20 +70,6 @@ __device__ __forceinline__ void tab_grad(const LaneTab &T, int x, int y, int z, 
   gr[2] = T.B[0][x] * T.B[1][y] * T.D[2][z];
 }
 
-// element of a parity class (color = px + 2 py + 4 pz) from its slot; no two
-// elements of a class share a node
-__device__ __forceinline__ long class_element(int mx, int my, int mz, int color, long slot, bool &live) {
-  const int px = color & 1, py = (color >> 1) & 1, pz = color >> 2;
-  const long ncx = (mx - px + 1) / 2, ncy = (my - py + 1) / 2, ncz = (mz - pz + 1) / 2;
-  live = slot < ncx * ncy * ncz;
-  if (!live) slot = ncx * ncy * ncz - 1;
-  return (2 * (slot % ncx) + px) + mx * ((2 * ((slot / ncx) % ncy) + py) + (long)my * (2 * (slot / (ncx * ncy)) + pz));
-}
-
-inline long class_count(const Box &b, int color) {
-  return (long)((b.mx - (color & 1) + 1) / 2) * ((b.my - ((color >> 1) & 1) + 1) / 2) * ((b.mz - (color >> 2) + 1) / 2);
-}
-
 __device__ void qp_geometry(const double (*xe)[3], int q, QpGeom &g) {
   const int qa = q % 3, qb = (q / 3) % 3, qc = q / 9;
   // the lane's rows of the 1-D tables, held in registers

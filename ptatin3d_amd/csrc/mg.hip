@@ -933,7 +933,7 @@ struct ptmf_mg_s {
   // (through ptmf_mg_level_op) makes the graph stale, and it is dropped
   struct GraphKey {
     unsigned long state = 0;
-    int TX = 0, TY = 0, ZC = 0, variant = 0;
+    int TX = 0, TY = 0, ZC = 0, variant = 0, wg = 1;
     const void *faces = nullptr, *bcidx = nullptr, *et64 = nullptr, *et32 = nullptr;
     long nbc = 0;
   } gkey[16];
@@ -2008,6 +2008,7 @@ void graph_key(const ptmf_op_s *op, ptmf_mg_s::GraphKey *out) {
   k.TY = op->tiling.TY;
   k.ZC = op->tiling.ZC;
   k.variant = op->variant;
+  k.wg = op->wg_tiles;
   k.faces = op->faces;
   k.bcidx = op->bcidx;
   k.et64 = op->et64;
@@ -2045,7 +2046,7 @@ bool graph_stale(const ptmf_mg_s *mg) {
     ptmf_mg_s::GraphKey k;
     graph_key(mg->lev[l].op, &k);
     const ptmf_mg_s::GraphKey &c = mg->gkey[l];
-    if (k.state != c.state || k.TX != c.TX || k.TY != c.TY || k.ZC != c.ZC || k.variant != c.variant ||
+    if (k.state != c.state || k.TX != c.TX || k.TY != c.TY || k.ZC != c.ZC || k.variant != c.variant || k.wg != c.wg ||
         k.faces != c.faces || k.bcidx != c.bcidx || k.et64 != c.et64 || k.et32 != c.et32 || k.nbc != c.nbc)
       return true;
   }

@@ -63,6 +63,22 @@ Tiling choose_tiling(int mx, int my, int mz);
 Tiling make_tiling(int mx, int my, int mz, int TX, int TY, int ZC);
 long face_buffer_doubles(int TX, int TY, int ZC, long ntiles);
 
+// Element of parity class `color` (= px + 2 py + 4 pz) at position `slot` of
+// the class, x fastest; no two elements of a class share a node, so their
+// sums can go into a vector with plain read-add-write.  live is false for
+// slots past the class (the last element is returned for them).
+__host__ __device__ inline long class_element(int mx, int my, int mz, int color, long slot, bool &live) {
+  const int px = color & 1, py = (color >> 1) & 1, pz = color >> 2;
+  const long ncx = (mx - px + 1) / 2, ncy = (my - py + 1) / 2, ncz = (mz - pz + 1) / 2;
+  live = slot < ncx * ncy * ncz;
+  if (!live) slot = ncx * ncy * ncz - 1;
+  return (2 * (slot % ncx) + px) + mx * ((2 * ((slot / ncx) % ncy) + py) + (long)my * (2 * (slot / (ncx * ncy)) + pz));
+}
+
+inline long class_count(const Box &b, int color) {
+  return (long)((b.mx - (color & 1) + 1) / 2) * ((b.my - ((color >> 1) & 1) + 1) / 2) * ((b.mz - (color >> 2) + 1) / 2);
+}
+
 // Per-precision device copies of the operator data.
 template <typename T>
 struct OpData {
@@ -86,7 +102,8 @@ struct ptmf_op_s {
   hipStream_t stream = nullptr;   // stream all work of this operator runs on
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ptmf::Tiling tiling;
-  int variant = 0;  // MatMult kernel: 0 z-march, 1 software-pipelined, 2 stored geometry
+  int variant = 0;  // MatMult kernel: 0 z-march, 1 software-pipelined, 2 stored geometry, 3 coloured
+  int wg_tiles = 1;  // tiles (one wave each) per work-group of the variant-0 z-march: 1, 2 or 4
   // stored quadrature-point geometry of variant 2 (per precision), valid for
   // coefficient state pa_state[i] and the tiling it was laid out for
   double *pa64 = nullptr;

@@ -84,6 +84,7 @@ class GMG:
 
     def setup(self, opt: MGOptions | None = None) -> None:
         call("ptmf_mg_setup", self.handle, C.byref(opt) if opt is not None else None)
+        self.opt = opt if opt is not None else default_options()
 
     def coarse_info(self) -> dict:
         """The coarse factor of the last set-up (dense inverse or sparse
@@ -216,13 +217,17 @@ def solve_timed(mg: GMG, b: np.ndarray, rtol=1e-8, maxit=200, restart=30, krylov
 
 
 def stokes_solve_timed(mg: GMG, bu: np.ndarray, bp: np.ndarray, rtol=1e-8, maxit=500, restart=50,
-                       history_len=1024):
-    """Saddle-point solve from x = 0; returns (u, p, its, rnorm, history, seconds)."""
+                       history_len=1024, x0=None):
+    """Saddle-point solve from x0 = [u | p] (default 0); returns (u, p, its,
+    rnorm, history, seconds)."""
     nu = bu.size
     b = np.concatenate([bu, bp])
     db = DeviceArray.from_host(b)
-    dx = DeviceArray(b.size)
-    dx.zero()
+    if x0 is None:
+        dx = DeviceArray(b.size)
+        dx.zero()
+    else:
+        dx = DeviceArray.from_host(np.asarray(x0, dtype=np.float64))
     call("ptmf_device_synchronize")
     t0 = time.perf_counter()
     its, rn, hist = mg.stokes_fgmres(db, dx, rtol, maxit, restart, history_len)

@@ -34,6 +34,19 @@ class MatA11MF:
         self.nel = mx * my * mz
         self.is_setup = False
 
+    @classmethod
+    def borrow(cls, handle, mx: int, my: int, mz: int) -> "MatA11MF":
+        """A view of an operator owned elsewhere (a multigrid level's, from
+        ptmf_mg_level_op); destroy() and garbage collection leave it alone."""
+        op = cls.__new__(cls)
+        op.mx, op.my, op.mz = int(mx), int(my), int(mz)
+        op.handle = handle
+        op.nnodes = (2 * mx + 1) * (2 * my + 1) * (2 * mz + 1)
+        op.nel = mx * my * mz
+        op.is_setup = True
+        op._owned = False
+        return op
+
     # -- MatA11MFSetup ----------------------------------------------------
     def setup(self, coords: np.ndarray, eta: np.ndarray, bcmask: np.ndarray) -> "MatA11MF":
         self.set_coordinates(coords)
@@ -67,8 +80,18 @@ class MatA11MF:
         return a.value, b.value, c.value
 
     def set_kernel_variant(self, variant: int) -> None:
-        """0: z-march; 1: z-march with next-layer loads issued one stage ahead."""
+        """0: z-march; 1: z-march with next-layer loads issued one stage ahead;
+        2: stored quadrature-point geometry; 3: coloured scatter (no tiles)."""
         call("ptmf_op_set_kernel_variant", self.handle, int(variant))
+
+    def set_tiles_per_group(self, tiles: int) -> None:
+        """Tiles (one wave each) per work-group of the variant-0 MatMult: 1, 2 or 4."""
+        call("ptmf_op_set_tiles_per_group", self.handle, int(tiles))
+
+    def get_tiles_per_group(self) -> int:
+        v = C.c_int()
+        call("ptmf_op_get_tiles_per_group", self.handle, C.byref(v))
+        return v.value
 
     def get_kernel_variant(self) -> int:
         v = C.c_int()
@@ -178,13 +201,14 @@ class MatA11MF:
         return a.value
 
     def destroy(self) -> None:
-        if self.handle is not None and self.handle.value:
+        if getattr(self, "_owned", True) and self.handle is not None and self.handle.value:
             call("ptmf_op_destroy", C.byref(self.handle))
         self.handle = None
 
     def __del__(self):
         try:
-            if self.handle is not None and self.handle.value and _lib._lib is not None:
+            if (getattr(self, "_owned", True) and self.handle is not None and self.handle.value
+                    and _lib._lib is not None):
                 _lib._lib.ptmf_op_destroy(C.byref(self.handle))
         except Exception:
             pass

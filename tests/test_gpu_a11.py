@@ -84,6 +84,35 @@ def test_tilings(gpu, tiling):
     A.set_kernel_variant(2)
     assert rel_inf(A.mult_host(x), yo) <= TOL64
     assert rel_inf(A.mult_host(x, precision=32), yo) <= TOL32
+    # coloured scatter (no tiles, sums added per element parity class)
+    A.set_kernel_variant(3)
+    assert rel_inf(A.mult_host(x), yo) <= TOL64
+    assert rel_inf(A.mult_host(x, precision=32), yo) <= TOL32
+    # two and four tiles per work-group: the same per-tile sums in another
+    # compiled kernel, so equal to rounding (waves of a group march through
+    # different numbers of layers where the z-chunks are ragged)
+    A.set_kernel_variant(0)
+    for w in (2, 4):
+        A.set_tiles_per_group(w)
+        assert A.get_tiles_per_group() == w
+        assert rel_inf(A.mult_host(x), y) <= 1e-14
+        assert rel_inf(A.mult_host(x, precision=32), y32) <= 1e-6
+    A.destroy()
+
+
+@pytest.mark.parametrize("m", [(1, 1, 1), (2, 1, 1), (3, 5, 7), (22, 1, 3), (16, 16, 16)])
+def test_coloured_scatter_variant(gpu, m):
+    """Variant 3 on ragged boxes (odd and even counts per parity class, one
+    element wide) and the 16^3 workload: the MatMult and, after it, the
+    Chebyshev consumer's form without the Dirichlet copy."""
+    coords, eta, bc, x = _case(m, seed=23)
+    A = _op(m, coords, eta, bc)
+    A.set_kernel_variant(3)
+    yo = O.a11_matmult(*m, coords, eta, bc, x)
+    y = A.mult_host(x)
+    assert rel_inf(y, yo) <= TOL64
+    assert np.array_equal(A.mult_host(x), y)  # fixed summation order
+    assert rel_inf(A.mult_host(x, precision=32), yo) <= TOL32
     A.destroy()
 
 

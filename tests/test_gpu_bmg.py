@@ -3,8 +3,10 @@ C-ABI: the batched V-cycle equals each box's own V-cycle (also on graph
 replays), and the batched flexible CG drives every box to its tolerance,
 checked with the oracle's operator on the host.
 
-Tolerances: V-cycle FP64 1e-10, FP32 1e-5 relative (the coarse products sum
-in another order than rocBLAS); solve: true FP64 residual <= 2e-8 ||b||.
+Tolerances: V-cycle FP64 1e-10, FP32 1e-5 relative (the batched coarse
+product sums in another order than the single-box one); solve: true FP64
+residual <= 2e-8 ||b||.  Log-uniform viscosity in [1e-3, 1e3] per quadrature
+point and box sides up to 24 elements, as the batched workload.
 """
 import numpy as np
 import pytest
@@ -18,7 +20,8 @@ from conftest import rel_inf
 
 pytestmark = pytest.mark.gpu
 
-DIMS = [(4, 4, 4), (8, 4, 8), (12, 8, 4), (8, 8, 8), (16, 8, 12), (4, 12, 8)]
+# sides from the batched workload's {4, 8, ..., 24}, one box 24 elements long
+DIMS = [(4, 4, 4), (8, 4, 8), (12, 8, 4), (8, 8, 8), (16, 8, 12), (4, 12, 8), (24, 8, 4)]
 
 
 def _boxes(prec, seed=7):
@@ -26,7 +29,7 @@ def _boxes(prec, seed=7):
     out = []
     for m in DIMS:
         coords = F.perturb_interior(F.box_coords(*m), *m, 0.1, rng)
-        eta = F.eta_loguniform(m[0] * m[1] * m[2], 1e-2, 1e2, rng)
+        eta = F.eta_loguniform(m[0] * m[1] * m[2], 1e-3, 1e3, rng)  # the workload's contrast 1e6
         mg = GMG(*m, 3)
         mg.set_coordinates(coords)
         mg.set_viscosity(2, eta)

@@ -348,3 +348,35 @@ def test_coarsen_viscosity_levels(gpu, mode):
         ref = O.a11_matmult(*dims, cl[l], etas[l], F.dirichlet_all_faces(*dims), x)
         assert rel_inf(dy.download(), ref) <= 1e-12, (mode, l)
     mg.destroy()
+
+
+def test_tune_hierarchy_keeps_the_vcycle(gpu, tmp_path, monkeypatch):
+    """Tuning every level's tiling and kernel variant (including the
+    coloured scatter) in the V-cycle's precision leaves the V-cycle the same
+    operator (to rounding) and is cached for the next call."""
+    from ptatin3d_amd import autotune
+
+    monkeypatch.setenv("PTMF_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_CACHE", {})
+    monkeypatch.setattr(autotune, "_LOADED", False)
+    m = (8, 8, 8)
+    mg, coords, eta = _hierarchy(m, 3, opt=default_options(smoother_precision=64))
+    ref, _, _ = _hierarchy(m, 3, opt=default_options(smoother_precision=64))
+    b = np.random.default_rng(3).uniform(-1, 1, 3 * F.nnodes(*m))
+    db = DeviceArray.from_host(b)
+    x0, x1 = DeviceArray(b.size), DeviceArray(b.size)
+    mg.vcycle(db, x0)  # captured graph from the second call on
+    mg.vcycle(db, x0)
+    choice = autotune.tune_hierarchy(mg, precision=64, reps=1, max_shapes=1)
+    assert [c["level"] for c in choice] == [0, 1, 2]
+    assert all(c["us"] is not None for c in choice)
+    for _ in range(3):  # stale graph dropped, eager cycle, new capture, replay
+        mg.vcycle(db, x0)
+        ref.vcycle(db, x1)
+        assert rel_inf(x0.download(), x1.download()) <= 1e-12
+    again = autotune.tune_hierarchy(mg, precision=64, reps=1, max_shapes=1)
+    assert [(c["tiling"], c["variant"], c["tiles_per_group"]) for c in again] == \
+        [(c["tiling"], c["variant"], c["tiles_per_group"]) for c in choice]
+    assert all(c["us"] is None for c in again)  # from the cache
+    mg.destroy()
+    ref.destroy()

@@ -120,3 +120,19 @@ def test_wrappers_refuse_mismatched_device_arrays():
             f()
         assert ei.value.code == 1
     assert isinstance(A, MatA11MF)
+
+
+def test_autotune_cache_persists(tmp_path, monkeypatch):
+    """Tuned choices survive the process: written to the JSON cache, read
+    back by a fresh module state, keyed by box, precision and candidates."""
+    monkeypatch.setenv("PTMF_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_CACHE", {})
+    monkeypatch.setattr(autotune, "_LOADED", True)
+    k = autotune._key(64, 64, 64, 32, (0, 1, 3), 4)
+    autotune._CACHE[k] = ((4, 5, 2), 3, 2)
+    autotune._save()
+    monkeypatch.setattr(autotune, "_CACHE", {})
+    monkeypatch.setattr(autotune, "_LOADED", False)
+    autotune._load()
+    assert autotune._CACHE[k] == ((4, 5, 2), 3, 2)
+    assert autotune._key(64, 64, 64, 64, (0, 1, 3), 4) not in autotune._CACHE
