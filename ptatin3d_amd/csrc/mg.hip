@@ -149,9 +149,11 @@ __global__ void k_cheb_first(long n, const T *__restrict__ b, const T *__restric
 // with bcn (per-node Dirichlet bits) the Dirichlet rows of A p_k are taken as
 // p_k (the MatMult's y = x), so Apk may hold the element sums there
 template <typename T>
+// pkp1 may be pkm1 (the step overwrites the iterate before last in place);
+// pkm1 == nullptr stands for a zero iterate
 __global__ void k_cheb_step(long n, const T *__restrict__ b, const T *__restrict__ Apk,
-                            const T *__restrict__ dinv, const T *__restrict__ pkm1,
-                            const T *__restrict__ pk, T os, T om1, T om, T *__restrict__ pkp1,
+                            const T *__restrict__ dinv, const T *pkm1,
+                            const T *__restrict__ pk, T os, T om1, T om, T *pkp1,
                             const uint8_t *__restrict__ bcn = nullptr) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     T ap = Apk[i];
@@ -160,7 +162,8 @@ __global__ void k_cheb_step(long n, const T *__restrict__ b, const T *__restrict
       if ((bcn[nd] >> (int)(i - 3 * nd)) & 1u) ap = pk[i];
     }
     const T z = (b[i] - ap) * dinv[i];
-    pkp1[i] = os * z + om1 * pkm1[i] + om * pk[i];
+    const T prev = pkm1 ? pkm1[i] : T(0);
+    pkp1[i] = os * z + om1 * prev + om * pk[i];
   }
 }
 
@@ -222,6 +225,15 @@ __global__ void k_dot3_partial(long n, const double *__restrict__ r, const doubl
 }
 
 // partial sums of one dot product <a, b>
+// the flexible-CG step x += alpha p, r -= alpha q in one pass
+__global__ void k_fcg_step(long n, double alpha, const double *__restrict__ p, const double *__restrict__ q,
+                           double *__restrict__ x, double *__restrict__ r) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    r[i] -= alpha * q[i];
+  }
+}
+
 __global__ void k_dot_partial(long n, const double *__restrict__ a, const double *__restrict__ b,
                               double *__restrict__ part) {
   __shared__ double red[kBS / 64];
@@ -726,14 +738,15 @@ __global__ void k_divide(long n, const double *__restrict__ a, const double *__r
 // sums are reduced through LDS.
 // kCheb: the Chebyshev-Jacobi step is applied to the product as it is
 // formed, pkp1 = os dinv (b - A x) + om1 pkm1 + om x (x = p_k), and y is not
-// written (the same arithmetic as the product followed by k_cheb_step).
+// written (the same arithmetic as the product followed by k_cheb_step);
+// pkp1 may be pkm1 (in place), pkm1 == nullptr stands for zero.
 constexpr int kStNodes = 16;
 template <typename T, bool kCheb = false>
 __global__ void __launch_bounds__(25 * kStNodes)
     k_stencil_apply(long X, long Y, long Z, const T *__restrict__ S, const T *__restrict__ x,
                     T *__restrict__ y, const T *__restrict__ b = nullptr, const T *__restrict__ dinv = nullptr,
-                    const T *__restrict__ pkm1 = nullptr, T os = T(0), T om1 = T(0), T om = T(0),
-                    T *__restrict__ pkp1 = nullptr) {
+                    const T *pkm1 = nullptr, T os = T(0), T om1 = T(0), T om = T(0),
+                    T *pkp1 = nullptr) {
   __shared__ T red[25][kStNodes][3];
   const long N = X * Y * Z;
   const int t = threadIdx.x;
@@ -769,7 +782,8 @@ __global__ void __launch_bounds__(25 * kStNodes)
     const long i = 3 * m + g;
     if constexpr (kCheb) {
       const T z = (b[i] - acc) * dinv[i];
-      pkp1[i] = os * z + om1 * pkm1[i] + om * x[i];
+      const T prev = pkm1 ? pkm1[i] : T(0);
+      pkp1[i] = os * z + om1 * prev + om * x[i];
     } else {
       y[i] = acc;
     }
@@ -923,10 +937,18 @@ struct ptmf_mg_s {
   double *sMinv = nullptr, *stmp = nullptr;
   double *sx2 = nullptr, *sy2 = nullptr;  // [u | p] temporaries of the preconditioner's A12 product
   bool stokes_ready = false;
-  // captured V-cycle (levels_mg == L): runs between the fixed buffers gb, gx
-  hipGraph_t vgraph = nullptr;
-  hipGraphExec_t vexec = nullptr;
-  double *gb = nullptr, *gx = nullptr;
+  // captured V-cycles (levels_mg == L), one per (b, x) pair of device
+  // pointers the cycle was called with: a replay reads b and writes x
+  // directly, no staging copies
+  static constexpr int kGraphs = 8;
+  struct VGraph {
+    const double *b = nullptr;
+    double *x = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    unsigned long used = 0;
+  } vg[kGraphs];
+  unsigned long vtick = 0;
   int vcalls = 0;
   // what each level's MatMult nodes were captured with: a later change of a
   // level operator's coefficients, tiling, kernel variant or Dirichlet list
@@ -939,7 +961,7 @@ struct ptmf_mg_s {
   } gkey[16];
   int s_inner_its = 1;       // A11^-1 in the preconditioner: 1 = one V-cycle, k > 1 = up to k
   double s_inner_rtol = 0.0; // flexible-CG iterations to relative tolerance s_inner_rtol
-  double *fcg[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // flexible CG workspace
+  double *fcg[4] = {nullptr, nullptr, nullptr, nullptr};  // flexible CG workspace: r, z, p, q
   std::vector<double> history;
 };
 
@@ -1064,45 +1086,50 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
   const double alpha = 1.0 - scale * emin;
   const double mu = 1.0 / alpha, omegaprod = 2.0 / alpha;
   double c[3];
-  // p[km1] = x, p[k] = first step
-  int km1 = 0, k = 1, kp1 = 2;
-  // copies and fills are kernels: the V-cycle may be captured into a graph,
-  // which then holds kernel nodes only
-  if (nonzero)
-    k_copy<T><<<nblocks(n), kBS, 0, s>>>(n, x, p[km1]);
-  else
-    PTMF_TRY(launch_fill<T>(p[km1], T(0), n, s));  // a kernel: the V-cycle may be captured
+  // Two iterates are live at a time: each step overwrites the iterate before
+  // last in place (kernels take pkp1 == pkm1), so the recurrence needs one
+  // buffer besides x.  A zero initial guess is no buffer at all (nullptr),
+  // and the steps alternate between x and p[0], so with an odd number of
+  // steps the last one lands in x: no fill, no copies (kernels only: the
+  // V-cycle may be captured into a graph).
+  T *prev = nullptr, *cur = p[0];
   if (nonzero) {
     PTMF_TRY(matmult<T>(L, x, r));
-    k_cheb_first<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, x, dinv, (T)scale, p[k]);
+    k_cheb_first<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, x, dinv, (T)scale, cur);
+    prev = x;
   } else {
-    k_cheb_first<T><<<nblocks(n), kBS, 0, s>>>(n, b, nullptr, nullptr, dinv, (T)scale, p[k]);
+    k_cheb_first<T><<<nblocks(n), kBS, 0, s>>>(n, b, nullptr, nullptr, dinv, (T)scale, cur);
   }
   PTMF_HIP(hipGetLastError());
-  c[km1] = 1.0;
-  c[k] = mu;
+  c[0] = 1.0;  // c[km1]
+  c[1] = mu;   // c[k]
   for (int i = 0; i < its; i++) {
-    c[kp1] = 2.0 * mu * c[k] - c[km1];
-    const double omega = omegaprod * c[k] / c[kp1];
+    c[2] = 2.0 * mu * c[1] - c[0];
+    const double omega = omegaprod * c[1] / c[2];
+    T *out = prev ? prev : x;
     if (L.galerkin) {
       // stencil product and Chebyshev update in one launch
       const Box &bx = L.op->box;
       const T *S = sizeof(T) == 8 ? (const T *)L.S64 : (const T *)L.S32;
       const long nb = (bx.nnodes + kStNodes - 1) / kStNodes;
       k_stencil_apply<T, true><<<dim3((unsigned)nb), dim3(25 * kStNodes), 0, L.op->stream>>>(
-          bx.X, bx.Y, bx.Z, S, p[k], r, b, dinv, p[km1], (T)(omega * scale), (T)(1.0 - omega), (T)omega, p[kp1]);
+          bx.X, bx.Y, bx.Z, S, cur, r, b, dinv, prev, (T)(omega * scale), (T)(1.0 - omega), (T)omega, out);
     } else {
       // MatMult without its Dirichlet-copy launch; the update takes y = x there
-      PTMF_TRY(launch_a11_mult<T>(L.op, p[k], r, 2));
-      k_cheb_step<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, dinv, p[km1], p[k], (T)(omega * scale),
-                                                 (T)(1.0 - omega), (T)omega, p[kp1], L.op->bcnode);
+      PTMF_TRY(launch_a11_mult<T>(L.op, cur, r, 2));
+      k_cheb_step<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, dinv, prev, cur, (T)(omega * scale), (T)(1.0 - omega),
+                                                 (T)omega, out, L.op->bcnode);
     }
     PTMF_HIP(hipGetLastError());
-    const int t = km1;
-    km1 = k; k = kp1; kp1 = t;
+    c[0] = c[1];
+    c[1] = c[2];
+    prev = cur;
+    cur = out;
   }
-  k_copy<T><<<nblocks(n), kBS, 0, s>>>(n, p[k], x);
-  PTMF_HIP(hipGetLastError());
+  if (cur != x) {
+    k_copy<T><<<nblocks(n), kBS, 0, s>>>(n, cur, x);
+    PTMF_HIP(hipGetLastError());
+  }
   return PTMF_OK;
 }
 
@@ -1422,12 +1449,21 @@ int setup_coarse(ptmf_mg_s *mg) {
   return st;
 }
 
+void drop_one_graph(ptmf_mg_s::VGraph &v) {
+  if (v.ex) (void)hipGraphExecDestroy(v.ex);
+  if (v.g) (void)hipGraphDestroy(v.g);
+  v = ptmf_mg_s::VGraph();
+}
+
 void drop_graph(ptmf_mg_s *mg) {
-  if (mg->vexec) (void)hipGraphExecDestroy(mg->vexec);
-  if (mg->vgraph) (void)hipGraphDestroy(mg->vgraph);
-  mg->vexec = nullptr;
-  mg->vgraph = nullptr;
+  for (auto &v : mg->vg) drop_one_graph(v);
   mg->vcalls = 0;
+}
+
+bool have_graph(const ptmf_mg_s *mg) {
+  for (const auto &v : mg->vg)
+    if (v.ex) return true;
+  return false;
 }
 
 // One reduction of the low-synchronisation Gram-Schmidt step for column k:
@@ -1762,8 +1798,6 @@ int ptmf_mg_destroy(ptmf_mg *pmg) {
   (void)hipFree(mg->sx2);
   (void)hipFree(mg->sy2);
   ptmf::drop_graph(mg);
-  (void)hipFree(mg->gb);
-  (void)hipFree(mg->gx);
   for (double *p : mg->pcg) (void)hipFree(p);
   for (double *p : mg->fcg) (void)hipFree(p);
   (void)hipFree(mg->red_part);
@@ -2016,18 +2050,26 @@ void graph_key(const ptmf_op_s *op, ptmf_mg_s::GraphKey *out) {
   k.nbc = op->nbc;
 }
 
-// Capture one V-cycle from gb into gx as a HIP graph.  Everything the cycle
-// builds lazily (tile-ordered data) exists after the first, eager call.
-int capture_vcycle(ptmf_mg_s *mg) {
-  const long n = mg->lev[mg->L - 1].n;
-  if (!mg->gb) PTMF_TRY(dalloc((void **)&mg->gb, sizeof(double) * n));
-  if (!mg->gx) PTMF_TRY(dalloc((void **)&mg->gx, sizeof(double) * n));
+// Capture one V-cycle from b into x as a HIP graph (into the least recently
+// used cache slot).  Everything the cycle builds lazily (tile-ordered data)
+// exists after the first, eager call.
+int capture_vcycle(ptmf_mg_s *mg, const double *b, double *x, ptmf_mg_s::VGraph **out) {
+  ptmf_mg_s::VGraph *slot = &mg->vg[0];
+  for (auto &v : mg->vg)
+    if (!v.ex || v.used < slot->used) {
+      slot = &v;
+      if (!v.ex) break;
+    }
+  drop_one_graph(*slot);
   PTMF_HIP(hipStreamSynchronize(mg->stream));
   PTMF_HIP(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeThreadLocal));
-  const int st = vcycle_body(mg, mg->gb, mg->gx);
+  const int st = vcycle_body(mg, b, x);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(mg->stream, &g);
-  PTMF_TRY(st);
+  if (st != PTMF_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
   PTMF_CHECK(ce == hipSuccess && g, PTMF_ERR_DEVICE, "mg_vcycle: graph capture failed: %s", hipGetErrorString(ce));
   hipGraphExec_t ex = nullptr;
   const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -2035,9 +2077,12 @@ int capture_vcycle(ptmf_mg_s *mg) {
     (void)hipGraphDestroy(g);
     return ptmf_set_error(PTMF_ERR_DEVICE, "mg_vcycle: graph instantiation failed: %s", hipGetErrorString(ie));
   }
-  mg->vgraph = g;
-  mg->vexec = ex;
+  slot->b = b;
+  slot->x = x;
+  slot->g = g;
+  slot->ex = ex;
   for (int l = 0; l < mg->L; l++) graph_key(mg->lev[l].op, &mg->gkey[l]);
+  *out = slot;
   return PTMF_OK;
 }
 
@@ -2065,13 +2110,14 @@ int ptmf_mg_vcycle(ptmf_mg mg, const double *b, double *x) {
   // a level operator changed since the capture: replay nothing stale, run
   // the next cycle eagerly (it rebuilds the operator's derived data) and
   // capture again after it
-  if (graph && mg->vexec && ptmf::graph_stale(mg)) ptmf::drop_graph(mg);
+  if (graph && ptmf::have_graph(mg) && ptmf::graph_stale(mg)) ptmf::drop_graph(mg);
   if (!graph || mg->vcalls++ == 0) return ptmf::vcycle_body(mg, b, x);
-  if (!mg->vexec) PTMF_TRY(ptmf::capture_vcycle(mg));
-  const long n = mg->lev[mg->L - 1].n;
-  PTMF_HIP(hipMemcpyAsync(mg->gb, b, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
-  PTMF_HIP(hipGraphLaunch(mg->vexec, mg->stream));
-  PTMF_HIP(hipMemcpyAsync(x, mg->gx, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+  ptmf_mg_s::VGraph *v = nullptr;
+  for (auto &c : mg->vg)
+    if (c.ex && c.b == b && c.x == x) v = &c;
+  if (!v) PTMF_TRY(ptmf::capture_vcycle(mg, b, x, &v));
+  v->used = ++mg->vtick;
+  PTMF_HIP(hipGraphLaunch(v->ex, mg->stream));
   return PTMF_OK;
 }
 
@@ -2144,7 +2190,7 @@ int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, 
   hipStream_t s = mg->stream;
   for (auto &p : mg->fcg)
     if (!p) PTMF_TRY(ptmf::dalloc((void **)&p, sizeof(double) * n));
-  double *r = mg->fcg[0], *z = mg->fcg[1], *p = mg->fcg[2], *q = mg->fcg[3], *rold = mg->fcg[4];
+  double *r = mg->fcg[0], *z = mg->fcg[1], *p = mg->fcg[2], *q = mg->fcg[3];
   double bb;
   PTMF_TRY(ptmf::dot(mg, n, b, b, &bb));
   const double bnorm = sqrt(bb);
@@ -2178,22 +2224,23 @@ int ptmf_mg_fcg(ptmf_mg mg, const double *b, double *x, double rtol, int maxit, 
       PTMF_TRY(ptmf::dot(mg, n, p, q, &pq));
       PTMF_CHECK(isfinite(pq) && pq != 0.0, PTMF_ERR_DIVERGED, "fcg: breakdown (p.Ap = %g)", pq);
       const double alpha = rz / pq;
-      PTMF_TRY(ptmf::axpby<double>(s, n, alpha, p, 1.0, x));
-      PTMF_HIP(hipMemcpyAsync(rold, r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-      PTMF_TRY(ptmf::axpby<double>(s, n, -alpha, q, 1.0, r));
-      // z = M r, then r.r, z.r and z.r_old in one fused reduction (one host
-      // synchronisation; the convergence test uses r.r)
+      ptmf::k_fcg_step<<<ptmf::nblocks(n), ptmf::kBS, 0, s>>>(n, alpha, p, q, x, r);
+      PTMF_HIP(hipGetLastError());
+      // z = M r, then r.r, z.r and z.q in one fused reduction (one host
+      // synchronisation; the convergence test uses r.r).  The flexible
+      // (Polak-Ribiere) beta = z.(r - r_old) / rz_old with r_old = r + alpha q
+      // is -alpha z.q / rz_old: no copy of the old residual.
       PTMF_TRY(ptmf_mg_vcycle(mg, r, z));
       double d3[3];
-      PTMF_TRY(ptmf::dot3(mg, n, r, z, rold, d3));
+      PTMF_TRY(ptmf::dot3(mg, n, r, z, q, d3));
       its++;
       rr = d3[0];
       rnorm = sqrt(rr);
       PTMF_CHECK(isfinite(rnorm), PTMF_ERR_DIVERGED, "fcg: non-finite residual");
       if (history && hl < history_len) history[hl++] = rnorm;
       if (rnorm <= rtol * bnorm) break;
-      const double zr = d3[1], zro = d3[2];
-      const double beta = (zr - zro) / rz;
+      const double zr = d3[1], zq = d3[2];
+      const double beta = -alpha * zq / rz;
       rz = zr;
       PTMF_TRY(ptmf::axpby<double>(s, n, 1.0, z, beta, p));
     }
