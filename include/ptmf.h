@@ -188,6 +188,18 @@ int ptmf_a11_spmv_host(ptmf_op op, const double *u, double *Yu);
 int ptmf_a11_mult(ptmf_op op, const double *x, double *y);
 /* FP32 version of ptmf_a11_mult (FP32 coordinates, viscosity and accumulation). */
 int ptmf_a11_mult_f32(ptmf_op op, const float *x, float *y);
+/* ptmf_a11_mult that allocates nothing (no hipMalloc / hipFree / allocating
+ * library call).  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  The z-march's face buffers live in a
+ * static device array of 64 MiB per GPU (8,388,608 doubles, held until the
+ * process ends): a box and tiling whose face buffers need more return
+ * PTMF_ERR_UNSUPPORTED.  Same kernels and results as ptmf_a11_mult (the
+ * stored-geometry and coloured kernel variants run as the default z-march).
+ * The operator must be set up (ptmf_op_set_viscosity / ptmf_op_set_tiling
+ * allocate its tile-ordered viscosity) and belong to the current device.
+ * All calls on one GPU share the array: calls that overlap on one GPU race. */
+int ptmf_a11_mult_noalloc(ptmf_op op, const double *x, double *y);
 /* wait for all work enqueued on the operator's stream */
 int ptmf_op_synchronize(ptmf_op op);
 /* MatGetDiagonal_MFStokes_A11: diag(A11), 1 in Dirichlet slots. */

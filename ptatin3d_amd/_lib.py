@@ -62,6 +62,7 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_a11_spmv_host": (_I, [_P, _P, _P]),
     "ptmf_a11_mult": (_I, [_P, _P, _P]),
     "ptmf_a11_mult_f32": (_I, [_P, _P, _P]),
+    "ptmf_a11_mult_noalloc": (_I, [_P, _P, _P]),
     "ptmf_op_synchronize": (_I, [_P]),
     "ptmf_a11_diagonal": (_I, [_P, _P]),
     "ptmf_a11_l1_diagonal": (_I, [_P, _P]),

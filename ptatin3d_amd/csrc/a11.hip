@@ -1309,17 +1309,48 @@ int ensure_pa(ptmf_op_s *op) {
   return PTMF_OK;
 }
 
-// The MatMult's kernels with the face buffers at fb (the operator's own, or
-// the static scratch of the allocation-free entry point).
+// The tile-ordered viscosity for precision T rebuilt, when stale, into the
+// buffer the operator already holds (kernel only, no allocation); an error
+// when that buffer does not exist or is too short for the current tiling.
 template <typename T>
-int launch_a11_mult_fb(ptmf_op_s *op, const T *x, T *y, int faces, T *fb) {
+int eta_tiled_in_place(ptmf_op_s *op) {
+  const int ip = sizeof(T) == 8 ? 0 : 1;
+  const Tiling &tl = op->tiling;
+  if (op->et_state[ip] == op->state && op->et_tiling[ip][0] == tl.TX && op->et_tiling[ip][1] == tl.TY &&
+      op->et_tiling[ip][2] == tl.ZC)
+    return PTMF_OK;
+  const long len = tl.ntiles * tl.ZC * 27L * kEPW;
+  T *dst = (T *)(sizeof(T) == 8 ? (void *)op->et64 : (void *)op->et32);
+  PTMF_CHECK(dst && len <= op->et_len, PTMF_ERR_NOT_SETUP,
+             "a11_mult_noalloc: the tile-ordered viscosity of the current tiling is not allocated "
+             "(ptmf_op_set_viscosity and ptmf_op_set_tiling build it; a hierarchy's level operator "
+             "gets it from its first ptmf_a11_mult)");
+  const TileArgs a = tile_args(op);
+  long nb = (len + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  eta_tile_kernel<T><<<dim3((unsigned)nb), dim3(256), 0, op->stream>>>(a, op->d64.eta, dst);
+  PTMF_HIP(hipGetLastError());
+  op->et_state[ip] = op->state;
+  op->et_tiling[ip][0] = tl.TX;
+  op->et_tiling[ip][1] = tl.TY;
+  op->et_tiling[ip][2] = tl.ZC;
+  return PTMF_OK;
+}
+
+// The MatMult's kernels with the face buffers at fb (the operator's own, or
+// the static scratch of the allocation-free entry point).  noalloc: nothing
+// is allocated on the way (the stored-geometry and coloured variants, which
+// build tables of their own or need no face buffers, give way to the z-march).
+template <typename T>
+int launch_a11_mult_fb(ptmf_op_s *op, const T *x, T *y, int faces, T *fb, bool noalloc = false) {
   const OpData<T> &d = *(const OpData<T> *)(sizeof(T) == 8 ? (const void *)&op->d64
                                                            : (const void *)&op->d32);
   const Tiling &tl = op->tiling;
   PTMF_CHECK(tl.TX * tl.TY <= kEPW && (2 * tl.TX + 1) * (2 * tl.TY + 1) <= kWinNodes && tl.ZC > 0,
              PTMF_ERR_ARG, "a11: invalid tiling %d x %d x %d", tl.TX, tl.TY, tl.ZC);
   PTMF_CHECK(fb != nullptr, PTMF_ERR_NOT_SETUP, "a11: face buffers not allocated");
-  if (op->variant == 3) {
+  const int variant = noalloc && op->variant >= 2 ? 0 : op->variant;
+  if (variant == 3) {
     // coloured scatter: no tiles, no face buffers
     const Box &b = op->box;
     PTMF_TRY(launch_fill<T>(y, T(0), 3 * b.nnodes, op->stream));
@@ -1336,17 +1367,17 @@ int launch_a11_mult_fb(ptmf_op_s *op, const T *x, T *y, int faces, T *fb) {
   const TileArgs a = tile_args(op);
   const long ntiles = tl.ntiles;
   T *fbx = fb, *fby = fbx + 3 * a.fbx_n * ntiles, *fbz = fby + 3 * a.fby_n * ntiles;
-  if (op->variant == 2) {
+  if (variant == 2) {
     PTMF_TRY(ensure_pa<T>(op));
     const T *pa = sizeof(T) == 8 ? (const T *)op->pa64 : (const T *)op->pa32;
     a11_tile_pa_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
         a, pa, op->bcnode, x, y, fbx, fby, fbz);
   } else {
-    PTMF_TRY(ensure_eta_tiled<T>(op));
+    PTMF_TRY(noalloc ? eta_tiled_in_place<T>(op) : ensure_eta_tiled<T>(op));
     const T *et = sizeof(T) == 8 ? (const T *)op->et64 : (const T *)op->et32;
     const int kw = op->wg_tiles;
     const long nwg = (ntiles + kw - 1) / kw;
-    if (op->variant == 1)
+    if (variant == 1)
       a11_tile_pf_kernel<T, true><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
           a, d.coords, et, op->bcnode, x, y, fbx, fby, fbz);
     else if (kw == 2)
@@ -1390,6 +1421,8 @@ int launch_stokes_mult(ptmf_op_s *op, const double *x, double *y) {
   PTMF_TRY(launch_bc_copy<double>(op->bcidx, op->nbc, x, y, op->stream));
   return PTMF_OK;
 }
+
+int prepare_eta_tiled(ptmf_op_s *op) { return ensure_eta_tiled<double>(op); }
 
 template int launch_a11<double>(const ptmf_op_s *, const double *, double *, bool);
 template int launch_a11<float>(const ptmf_op_s *, const float *, float *, bool);
@@ -1613,3 +1646,64 @@ int ptmf_batch_time(ptmf_batch b, int precision, const void *const *x, void *con
 
 }  // extern "C"
 
+
+// ---------------------------------------------------------------------------
+// Allocation-free MatMult (ptmf_a11_mult_noalloc): the FP64 MatMult of
+// ptmf_a11_mult with the z-march's face buffers in `scratch`, a file-scope
+// device array of kScratchDoubles doubles (64 MiB) that every GPU the library
+// runs on holds until the process ends, instead of the operator's own
+// allocation.  In host code `scratch` names a host-side stand-in, not device
+// memory: the device address comes from hipGetSymbolAddress, once per
+// device, with that device current, and only that address reaches kernels.
+// Every call on one GPU shares its array (the tile kernel writes every face
+// entry the face kernel reads, so nothing needs clearing): calls that overlap
+// on one GPU race.
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+namespace {
+constexpr long kScratchDoubles = 8L << 20;
+constexpr int kScratchDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex scratch_mu;
+double *scratch_at[kScratchDevices];
+
+// device address of `scratch` on device dev (the current device)
+int scratch_address(int dev, double **out) {
+  PTMF_CHECK(dev >= 0 && dev < kScratchDevices, PTMF_ERR_DEVICE, "a11_mult_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_at[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    PTMF_CHECK(e == hipSuccess && p, PTMF_ERR_DEVICE, "a11_mult_noalloc: hipGetSymbolAddress(scratch): %s",
+               hipGetErrorString(e));
+    scratch_at[dev] = (double *)p;
+  }
+  *out = scratch_at[dev];
+  return PTMF_OK;
+}
+}  // namespace
+
+extern "C" int ptmf_a11_mult_noalloc(ptmf_op op, const double *x, double *y) {
+  PTMF_CHECK(op, PTMF_ERR_ARG, "a11_mult_noalloc: null operator");
+  PTMF_CHECK(op->has_coords && op->has_eta && op->has_bc, PTMF_ERR_NOT_SETUP, "MF operator not setup");
+  PTMF_CHECK(x && y && (const void *)x != (const void *)y, PTMF_ERR_ARG,
+             "a11_mult_noalloc: x and y must be distinct device arrays");
+  int dev = -1;
+  PTMF_HIP(hipGetDevice(&dev));
+  PTMF_CHECK(dev == op->device, PTMF_ERR_ARG,
+             "a11_mult_noalloc: the operator lives on device %d but device %d is current", op->device, dev);
+  const ptmf::Tiling &tl = op->tiling;
+  const long need = ptmf::face_buffer_doubles(tl.TX, tl.TY, tl.ZC, tl.ntiles);
+  PTMF_CHECK(need <= kScratchDoubles, PTMF_ERR_UNSUPPORTED,
+             "a11_mult_noalloc: the face buffers of this box and tiling take %ld doubles, the static scratch "
+             "holds %ld (64 MiB); use ptmf_a11_mult",
+             need, kScratchDoubles);
+  double *fb = nullptr;
+  PTMF_TRY(scratch_address(dev, &fb));
+  return ptmf::launch_a11_mult_fb<double>(op, x, y, 1, fb, true);
+}

@@ -229,7 +229,7 @@ int ptmf_op_set_tiling(ptmf_op op, int TX, int TY, int ZC) {
     op->faces_doubles = need;
   }
   op->tiling = t;
-  return PTMF_OK;
+  return op->has_eta ? ptmf::prepare_eta_tiled(op) : PTMF_OK;
 }
 
 int ptmf_op_set_kernel_variant(ptmf_op op, int variant) {
@@ -312,15 +312,18 @@ int ptmf_op_set_viscosity(ptmf_op op, const double *eta) {
   PTMF_HIP(hipMemcpy(op->d32.eta, t32.data(), sizeof(float) * 27 * nel, hipMemcpyHostToDevice));
   op->has_eta = 1;
   op->state++;
-  return PTMF_OK;
+  // the MatMult's tile-ordered copy is built here, at set-up, so that the
+  // allocation-free MatMult finds it allocated
+  return ptmf::prepare_eta_tiled(op);
 }
 
 int ptmf_op_set_viscosity_device(ptmf_op op, const double *eta) {
   PTMF_CHECK(op && eta, PTMF_ERR_ARG, "set_viscosity_device: null");
   PTMF_TRY(ptmf::launch_eta_qpmajor(op->box.nel, eta, op->d64.eta, op->d32.eta, op->stream));
-  PTMF_HIP(hipStreamSynchronize(op->stream));
   op->has_eta = 1;
   op->state++;
+  PTMF_TRY(ptmf::prepare_eta_tiled(op));
+  PTMF_HIP(hipStreamSynchronize(op->stream));
   return PTMF_OK;
 }
 

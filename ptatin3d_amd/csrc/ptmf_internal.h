@@ -157,6 +157,9 @@ int launch_a11(const ptmf_op_s *op, const T *u, T *y, bool mask_columns);
 // that takes y = x there itself.
 template <typename T>
 int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, int faces);
+// build (or rebuild) the FP64 tile-ordered viscosity for the current
+// coefficients and tiling (allocates; set-up time)
+int prepare_eta_tiled(ptmf_op_s *op);
 template <typename T>
 int launch_fill(T *x, T v, long n, hipStream_t s);
 template <typename T>
