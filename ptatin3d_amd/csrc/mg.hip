@@ -134,12 +134,17 @@ __global__ void k_recip(long n, const double *__restrict__ d, T *__restrict__ di
 }
 
 // first Chebyshev step: p = x + scale * dinv * (b - Ax)   (Ax == nullptr: x = 0)
-template <typename T>
-__global__ void k_cheb_first(long n, const T *__restrict__ b, const T *__restrict__ Ax,
+// b may be of a wider type Tb than T (the FP64 right-hand side of an FP32
+// V-cycle): it is rounded to T first and, with bout, kept in T for the
+// later steps (the cast and the first step in one pass).
+template <typename T, typename Tb = T>
+__global__ void k_cheb_first(long n, const Tb *__restrict__ b, const T *__restrict__ Ax,
                              const T *__restrict__ x, const T *__restrict__ dinv, T scale,
-                             T *__restrict__ p) {
+                             T *__restrict__ p, T *__restrict__ bout = nullptr) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const T r = Ax ? b[i] - Ax[i] : b[i];
+    const T bi = (T)b[i];
+    if (bout) bout[i] = bi;
+    const T r = Ax ? bi - Ax[i] : bi;
     const T x0 = x ? x[i] : T(0);
     p[i] = x0 + scale * (r * dinv[i]);
   }
@@ -148,12 +153,14 @@ __global__ void k_cheb_first(long n, const T *__restrict__ b, const T *__restric
 // p_kp1 = (omega scale) dinv (b - A p_k) + (1 - omega) p_km1 + omega p_k;
 // with bcn (per-node Dirichlet bits) the Dirichlet rows of A p_k are taken as
 // p_k (the MatMult's y = x), so Apk may hold the element sums there
-template <typename T>
 // pkp1 may be pkm1 (the step overwrites the iterate before last in place);
-// pkm1 == nullptr stands for a zero iterate
+// pkm1 == nullptr stands for a zero iterate.  To: the type the new iterate
+// is stored in (the FP64 output of an FP32 V-cycle's last step: the step
+// and the cast in one pass).
+template <typename T, typename To = T>
 __global__ void k_cheb_step(long n, const T *__restrict__ b, const T *__restrict__ Apk,
                             const T *__restrict__ dinv, const T *pkm1,
-                            const T *__restrict__ pk, T os, T om1, T om, T *pkp1,
+                            const T *__restrict__ pk, T os, T om1, T om, To *pkp1,
                             const uint8_t *__restrict__ bcn = nullptr) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     T ap = Apk[i];
@@ -163,7 +170,7 @@ __global__ void k_cheb_step(long n, const T *__restrict__ b, const T *__restrict
     }
     const T z = (b[i] - ap) * dinv[i];
     const T prev = pkm1 ? pkm1[i] : T(0);
-    pkp1[i] = os * z + om1 * prev + om * pk[i];
+    pkp1[i] = (To)(os * z + om1 * prev + om * pk[i]);
   }
 }
 
@@ -1067,9 +1074,15 @@ int restrict_(ptmf_mg_s *mg, int l, const T *xf, T *yc) {
   return PTMF_OK;
 }
 
-// Chebyshev-Jacobi smoothing on level L (precision T): x <- cheb(b, x)
+// Chebyshev-Jacobi smoothing on level L (precision T): x <- cheb(b, x).
+// b64 (zero initial guess only): the right-hand side arrives in FP64 and the
+// first step also writes its rounding into b, which the later steps read.
+// x64: the result goes to x64 in FP64 (the last step stores it) and x is not
+// written.  Both fold the precision casts of an FP32 V-cycle's finest level
+// into the smoother's first and last kernels.
 template <typename T>
-int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
+int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero, const double *b64 = nullptr,
+         double *x64 = nullptr) {
   hipStream_t s = mg->stream;
   const long n = L.n;
   T *p[3], *r;
@@ -1093,10 +1106,13 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
   // steps the last one lands in x: no fill, no copies (kernels only: the
   // V-cycle may be captured into a graph).
   T *prev = nullptr, *cur = p[0];
+  PTMF_CHECK(!b64 || !nonzero, PTMF_ERR_ARG, "cheb: an FP64 right-hand side needs a zero initial guess");
   if (nonzero) {
     PTMF_TRY(matmult<T>(L, x, r));
     k_cheb_first<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, x, dinv, (T)scale, cur);
     prev = x;
+  } else if (b64) {
+    k_cheb_first<T, double><<<nblocks(n), kBS, 0, s>>>(n, b64, nullptr, nullptr, dinv, (T)scale, cur, (T *)b);
   } else {
     k_cheb_first<T><<<nblocks(n), kBS, 0, s>>>(n, b, nullptr, nullptr, dinv, (T)scale, cur);
   }
@@ -1117,6 +1133,13 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
     } else {
       // MatMult without its Dirichlet-copy launch; the update takes y = x there
       PTMF_TRY(launch_a11_mult<T>(L.op, cur, r, 2));
+      if (x64 && i == its - 1) {
+        // the last step stores the result in FP64
+        k_cheb_step<T, double><<<nblocks(n), kBS, 0, s>>>(n, b, r, dinv, prev, cur, (T)(omega * scale),
+                                                           (T)(1.0 - omega), (T)omega, x64, L.op->bcnode);
+        PTMF_HIP(hipGetLastError());
+        return PTMF_OK;
+      }
       k_cheb_step<T><<<nblocks(n), kBS, 0, s>>>(n, b, r, dinv, prev, cur, (T)(omega * scale), (T)(1.0 - omega),
                                                  (T)omega, out, L.op->bcnode);
     }
@@ -1126,6 +1149,7 @@ int cheb(ptmf_mg_s *mg, MGLevel &L, const T *b, T *x, int its, bool nonzero) {
     prev = cur;
     cur = out;
   }
+  if (x64) return cast<T, double>(s, n, cur, x64);
   if (cur != x) {
     k_copy<T><<<nblocks(n), kBS, 0, s>>>(n, cur, x);
     PTMF_HIP(hipGetLastError());
@@ -1151,9 +1175,12 @@ int vcycle64(ptmf_mg_s *mg, int l, int lc, const double *b, double *x);
 int coarse_solve(ptmf_mg_s *mg, int lc, const double *b, double *x);
 
 // V-cycle in precision T from level l down to the coarse-solve level lc;
-// x is overwritten (zero initial guess), b is the right-hand side.
+// x is overwritten (zero initial guess), b is the right-hand side.  b64 /
+// x64 (level l only): the right-hand side arrives and the result leaves in
+// FP64, with b and x the level's buffers in T (the casts are folded into the
+// smoother's first and last kernels).
 template <typename T>
-int vcycle(ptmf_mg_s *mg, int l, int lc, const T *b, T *x) {
+int vcycle(ptmf_mg_s *mg, int l, int lc, const T *b, T *x, const double *b64 = nullptr, double *x64 = nullptr) {
   MGLevel &L = mg->lev[l];
   hipStream_t s = mg->stream;
   if (l == lc) {
@@ -1168,7 +1195,7 @@ int vcycle(ptmf_mg_s *mg, int l, int lc, const T *b, T *x) {
   T *bc = sizeof(T) == 8 ? (T *)C.b64 : (T *)C.b32;
   T *xc = sizeof(T) == 8 ? (T *)C.x64 : (T *)C.x32;
   // pre-smooth from zero
-  PTMF_TRY(cheb<T>(mg, L, b, x, mg->opt.pre_its, false));
+  PTMF_TRY(cheb<T>(mg, L, b, x, mg->opt.pre_its, false, b64));
   // residual and restriction
   PTMF_TRY(matmult<T>(L, x, r));
   k_residual<T><<<nblocks(L.n), kBS, 0, s>>>(L.n, b, r, r);
@@ -1181,7 +1208,7 @@ int vcycle(ptmf_mg_s *mg, int l, int lc, const T *b, T *x) {
   PTMF_TRY(vcycle<T>(mg, l - 1, lc, bc, xc));
   PTMF_TRY(prolong_add<T>(mg, l, xc, x));
   // post-smooth from the corrected iterate
-  PTMF_TRY(cheb<T>(mg, L, b, x, mg->opt.post_its, true));
+  PTMF_TRY(cheb<T>(mg, L, b, x, mg->opt.post_its, true, nullptr, x64));
   return PTMF_OK;
 }
 
@@ -2030,9 +2057,7 @@ int vcycle_body(ptmf_mg_s *mg, const double *b, double *x) {
   const int lf = mg->L - 1, lc = mg->L - mg->opt.levels_mg;
   MGLevel &F = mg->lev[lf];
   if (mg->opt.smoother_precision == 64 || lf == lc) return vcycle<double>(mg, lf, lc, b, x);
-  PTMF_TRY((cast<double, float>(mg->stream, F.n, b, F.b32)));
-  PTMF_TRY(vcycle<float>(mg, lf, lc, F.b32, F.x32));
-  return cast<float, double>(mg->stream, F.n, F.x32, x);
+  return vcycle<float>(mg, lf, lc, F.b32, F.x32, b, x);
 }
 
 void graph_key(const ptmf_op_s *op, ptmf_mg_s::GraphKey *out) {

@@ -380,3 +380,85 @@ def test_tune_hierarchy_keeps_the_vcycle(gpu, tmp_path, monkeypatch):
     assert all(c["us"] is None for c in again)  # from the cache
     mg.destroy()
     ref.destroy()
+
+
+def _probe_diagonal(mg, level):
+    """diag of a level operator from 375 coloured probes of the FP64 level
+    product (nodes of one colour are 5 apart, beyond the +-2 coupling)."""
+    mx, my, mz = mg.dims[level]
+    X, Y, Z = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+    n = 3 * X * Y * Z
+    k, j, i = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    i, j, k = i.reshape(-1), j.reshape(-1), k.reshape(-1)
+    d = np.zeros(n)
+    dv, dy = DeviceArray(n), DeviceArray(n)
+    for ci in range(5):
+        for cj in range(5):
+            for ck in range(5):
+                nodes = np.nonzero((i % 5 == ci) & (j % 5 == cj) & (k % 5 == ck))[0]
+                for comp in range(3):
+                    v = np.zeros(n)
+                    v[3 * nodes + comp] = 1.0
+                    dv.upload(v)
+                    mg.level_apply(level, dv, dy)
+                    d[3 * nodes + comp] = dy.download()[3 * nodes + comp]
+    dv.free()
+    dy.free()
+    return d
+
+
+def _host_chebyshev(mg, level, d, b, x0, its):
+    """The Chebyshev-Jacobi recurrence with unfused FP64 level products."""
+    emin, emax = mg.eigs(level)
+    scale = 2.0 / (emax + emin)
+    alpha = 1.0 - scale * emin
+    mu, omegaprod = 1.0 / alpha, 2.0 / alpha
+    dv, dy = DeviceArray(b.size), DeviceArray(b.size)
+
+    def A(v):
+        dv.upload(v)
+        mg.level_apply(level, dv, dy)
+        return dy.download()
+
+    xk0 = np.zeros_like(b) if x0 is None else x0.copy()
+    r0 = b - A(xk0) if x0 is not None else b
+    pkm1, pk = xk0, xk0 + scale * r0 / d
+    ckm1, ck = 1.0, mu
+    for _ in range(its):
+        ckp1 = 2.0 * mu * ck - ckm1
+        om = omegaprod * ck / ckp1
+        pkp1 = om * scale * (b - A(pk)) / d + (1.0 - om) * pkm1 + om * pk
+        pkm1, pk, ckm1, ck = pk, pkp1, ck, ckp1
+    dv.free()
+    dy.free()
+    return pk
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fused_chebyshev_matches_unfused_recurrence(gpu, prec):
+    """ptmf_mg_smooth on a Galerkin level (stencil product with the update
+    inside it) and a rediscretised level (MatMult without the Dirichlet copy,
+    update taking y = x there; in-place iterates), odd and even step counts,
+    zero and nonzero initial guess, FP64 and FP32, against the recurrence on
+    the unfused FP64 level products: 1e-12 (FP64), 2e-5 (FP32)."""
+    m = (8, 8, 4)
+    mg, coords, eta = _hierarchy(m, 3, opt=default_options(galerkin_levels=2))
+    rng = np.random.default_rng(21)
+    dt = np.float64 if prec == 64 else np.float32
+    tol = 1e-12 if prec == 64 else 2e-5
+    for level in (1, 2):  # level 1 Galerkin, level 2 rediscretised
+        d = _probe_diagonal(mg, level)
+        n = d.size
+        b = rng.uniform(-1, 1, n)
+        x0 = rng.uniform(-1, 1, n)
+        for its in (3, 4):
+            for nonzero in (False, True):
+                ref = _host_chebyshev(mg, level, d, b, x0 if nonzero else None, its)
+                db = DeviceArray.from_host(b.astype(dt))
+                dx = DeviceArray.from_host((x0 if nonzero else np.zeros(n)).astype(dt))
+                mg.smooth(level, db, dx, its, nonzero)
+                got = dx.download().astype(np.float64)
+                assert rel_inf(got, ref) <= tol, (level, its, nonzero, rel_inf(got, ref))
+                db.free()
+                dx.free()
+    mg.destroy()
