@@ -7,21 +7,27 @@ Dirichlet rows on all faces.  One step = one MatMult with vectors resident in
 HBM (column masking, element apply, Dirichlet rows).  Prints one JSON line.
 
 The MatMult tiling and kernel variant are picked by the autotuner before the
-timed region.  The "batch" object times the batched MatMult of --batch
-independent boxes (default 256, 0 skips it) against one MatMult per box.  The "solve" object is the mixed-precision multigrid solve of
-the layered 1e6-contrast workload (FP32 V-cycle vs FP64 V-cycle, flexible CG
-in FP64, both to a 1e-8 relative FP64 residual), at --solve-m elements per
-direction (default 64; --no-solve skips it).  --copy-probe runs one plain
-device copy of 256 MiB after the timed region (the reference kernel for HBM
-byte counters).  Opt-in legs: --batch-solve N (batched multigrid solve of N
-boxes) and --stokes MX MY MZ (the rift saddle point: fused MatMult rate, then
-FP32- and FP64-V-cycle FGMRES solves to 1e-8; --stokes-schur mass|bdb,
---stokes-restart, --stokes-maxit).
+timed region.  Further objects of the line:
+  batch   the batched MatMult of --batch independent boxes (default 256, 0
+          skips it) against one MatMult per box;
+  solve   the time-to-1e-8 solve of the A11 block on the layered
+          contrast-1e6 workload at --solve-m elements per direction (default
+          128): flexible CG in FP64 with a 4-level V-cycle (Chebyshev(3),
+          exact FP64 coarse factor), FP32 V-cycle against FP64 V-cycle, both
+          to a 1e-8 FP64 residual and compared with a 1e-11 FP64 solve
+          (--no-solve skips it);
+  stokes  the rift saddle point (--stokes MX MY MZ, default 192 192 96;
+          --no-stokes skips it): fused MatMult rate, then FP32- and
+          FP64-V-cycle FGMRES solves to 1e-8 and their error against a 1e-11
+          FP64 solve (--stokes-schur mass|bdb, --stokes-restart,
+          --stokes-maxit).
+--copy-probe runs one plain device copy of 256 MiB after the timed region
+(the reference kernel for HBM byte counters).  Opt-in: --batch-solve N
+(batched multigrid solve of N boxes).
 
   python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--m 64] [--no-cpu]
-                  [--solve-m 64 | --no-solve] [--copy-probe]
-                  [--batch-solve 256] [--stokes 192 192 96 [--stokes-schur bdb]
-                  [--stokes-restart 100] [--stokes-maxit 1200]]
+                  [--solve-m 128 | --no-solve] [--copy-probe] [--batch-solve 256]
+                  [--stokes 192 192 96 | --no-stokes] [--stokes-schur bdb]
 """
 from __future__ import annotations
 
@@ -82,16 +88,20 @@ def main() -> int:
                     help="outer Krylov method of the solve leg (FGMRES(30) stagnates near 0.9 on this workload)")
     ap.add_argument("--solve-restart", type=int, default=30)
     ap.add_argument("--solve-galerkin", type=int, default=0, help="Galerkin coarse levels of the solve leg")
+    ap.add_argument("--solve-maxit", type=int, default=4000,
+                    help="iteration cap of each solve of the solve leg (a few for a profiled run)")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--batch", type=int, default=256, help="boxes in the batched-MatMult leg (0: skip)")
     ap.add_argument("--batch-solve", type=int, default=0,
                     help="boxes in the batched GMG solve leg (0: skip; the configured workload uses 256)")
-    ap.add_argument("--stokes", type=int, nargs=3, metavar=("MX", "MY", "MZ"), default=None,
-                    help="run the rift saddle-point leg on this box (e.g. 192 192 96)")
-    ap.add_argument("--stokes-maxit", type=int, default=600)
-    ap.add_argument("--stokes-restart", type=int, default=40, help="FGMRES restart length of the rift leg")
-    ap.add_argument("--stokes-schur", choices=("mass", "bdb"), default="mass",
-                    help="Schur approximation of the rift leg: scaled viscous mass or B^T diag(A11)^-1 B blocks")
+    ap.add_argument("--stokes", type=int, nargs=3, metavar=("MX", "MY", "MZ"), default=[192, 192, 96],
+                    help="box of the rift saddle-point leg (default 192 192 96)")
+    ap.add_argument("--no-stokes", action="store_true", help="skip the rift saddle-point leg")
+    ap.add_argument("--stokes-maxit", type=int, default=1200)
+    ap.add_argument("--stokes-restart", type=int, default=100, help="FGMRES restart length of the rift leg")
+    ap.add_argument("--stokes-schur", choices=("mass", "bdb"), default="bdb",
+                    help="Schur approximation of the rift leg: B^T diag(A11)^-1 B element blocks (default) or the "
+                         "reference's scaled viscous mass (FGMRES stalls near 3e-7 with it on this workload)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -250,10 +260,10 @@ def main() -> int:
         out["batch"] = batch_leg(args.batch)
     if not args.no_solve:
         out["solve"] = solve_leg(args.solve_m, args.solve_levels, args.solve_cheb, args.solve_krylov,
-                                 args.solve_restart, args.solve_galerkin)
+                                 args.solve_restart, args.solve_galerkin, args.solve_maxit)
     if args.batch_solve > 0:
         out["batch_solve"] = batch_solve_leg(args.batch_solve)
-    if args.stokes:
+    if args.stokes and not args.no_stokes:
         out["stokes"] = stokes_leg(tuple(args.stokes), args.stokes_maxit, args.stokes_schur, args.stokes_restart)
     print(json.dumps(out), flush=True)
     return 0
@@ -542,6 +552,10 @@ def stokes_leg(dims, maxit: int, schur: str = "mass", restart: int = 40) -> dict
            "preconditioner": "upper block-triangular: " + {"mass": "scaled viscous-mass",
                                                             "bdb": "B^T diag(A11)^-1 B element-block"}[schur]
                              + " Schur, one V-cycle for A11"}
+    if schur == "bdb":
+        res["deviation"] = ("Schur approximation: element blocks of B^T diag(A11)^-1 B (no counterpart in the "
+                            "reference) instead of the reference's scaled viscous mass, with which FGMRES stalls "
+                            "near 3e-7 on this workload")
     # fused saddle-point MatMult throughput
     A = MatStokesMF(mx, my, mz).setup(c["coords"], eta, c["bc"])
     x = DeviceArray.from_host(np.concatenate([bu, bp]))
