@@ -22,11 +22,12 @@ timed region.  Further objects of the line:
           FP64 solve (--stokes-schur mass|bdb, --stokes-restart,
           --stokes-maxit).
 --copy-probe runs one plain device copy of 256 MiB after the timed region
-(the reference kernel for HBM byte counters).  Opt-in: --batch-solve N
-(batched multigrid solve of N boxes).
+(the reference kernel for HBM byte counters).  batch_solve: the batched
+multigrid solve of --batch-solve boxes (default 256; 0 skips it) against
+each box solved on its own.
 
   python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--m 64] [--no-cpu]
-                  [--solve-m 128 | --no-solve] [--copy-probe] [--batch-solve 256]
+                  [--solve-m 128 | --no-solve] [--copy-probe] [--batch-solve 256 | --batch-solve 0]
                   [--stokes 192 192 96 | --no-stokes] [--stokes-schur bdb]
 """
 from __future__ import annotations
@@ -94,8 +95,8 @@ def main() -> int:
                     help="iteration cap of each solve of the solve leg (a few for a profiled run)")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--batch", type=int, default=256, help="boxes in the batched-MatMult leg (0: skip)")
-    ap.add_argument("--batch-solve", type=int, default=0,
-                    help="boxes in the batched GMG solve leg (0: skip; the configured workload uses 256)")
+    ap.add_argument("--batch-solve", type=int, default=256,
+                    help="boxes in the batched GMG solve leg (default 256, the configured workload; 0 skips it)")
     ap.add_argument("--stokes", type=int, nargs=3, metavar=("MX", "MY", "MZ"), default=[192, 192, 96],
                     help="box of the rift saddle-point leg (default 192 192 96)")
     ap.add_argument("--no-stokes", action="store_true", help="skip the rift saddle-point leg")
@@ -155,6 +156,45 @@ def main() -> int:
     tflops = flops / (ms_kernel * 1e-3) / 1e12
     gbs = alg_bytes / (ms_kernel * 1e-3) / 1e9
 
+    kname = {1: "a11_tile_pf_kernel<double, true>", 3: "a11_colour_kernel<double>"}.get(
+        variant, "a11_tile_kernel<double, true>" if wg_tiles == 1 else f"a11_tile_wg_kernel<double, true, {wg_tiles}>")
+
+    # the FP32 MatMult of the V-cycle levels on the same workload, tuned in
+    # its own precision (reported beside; the value is the FP64 MatMult)
+    x32 = DeviceArray.from_host(np.asarray(c["u"], dtype=np.float32))
+    y32 = DeviceArray(ndof, np.float32)
+    if args.tiling:
+        pass
+    elif not args.no_tune:
+        autotune.tune(A, precision=32, reps=10, max_shapes=4, variants=(0, 1, autotune.COLOUR))
+    A.time_mult(x32, y32, max(args.warmup, 1))
+    ms32 = A.time_mult(x32, y32, args.steps)
+    ms32_kernel = A.time_kernel(x32, y32, args.steps)
+    A.mult(x32, y32)
+    err32 = float(np.abs(y32.download().astype(np.float64) - y_gpu).max() / np.abs(y_gpu).max())
+    tflops32 = flops / (ms32_kernel * 1e-3) / 1e12
+    v32, w32 = A.get_kernel_variant(), A.get_tiles_per_group()
+    fp32 = {
+        "tiling": list(A.get_tiling()), "kernel_variant": v32, "tiles_per_group": w32,
+        "ms_matmult": round(ms32, 5), "GDoF_s": round(ndof / (ms32 * 1e-3) / 1e9, 3),
+        "kernel": {1: "a11_tile_pf_kernel<float, true>", 3: "a11_colour_kernel<float>"}.get(
+            v32, "a11_tile_kernel<float, true>" if w32 == 1 else f"a11_tile_wg_kernel<float, true, {w32}>"),
+        "kernel_ms": round(ms32_kernel, 5),
+        "achieved_TFLOPs": round(tflops32, 3),
+        "peak_TFLOPs": 157.3,
+        "frac": round(tflops32 / 157.3, 4),
+        "frac_of_nonpacked_vector_fp32": round(tflops32 / PEAK_FP64_TFLOPS, 4),
+        "note": "packed FP32 instructions are disabled in this build, so the FP32 VALU ceiling is the "
+                "non-packed 78.6 TFLOP/s; 157.3 is MI355X's published FP32 vector peak",
+        "rel_inf_vs_fp64": err32,
+    }
+    x32.free()
+    y32.free()
+    if not args.tiling and not args.no_tune:
+        A.set_tiling(*tiling)
+        A.set_kernel_variant(variant)
+        A.set_tiles_per_group(wg_tiles)
+
     # diagonal (setup-time companion of the apply in this workload)
     d = DeviceArray(ndof)
     A.get_diagonal(d)  # first call: set-up of the constant tables
@@ -204,8 +244,7 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": round(tflops / PEAK_FP64_TFLOPS, 4),
             "traffic": MEASURED_TRAFFIC.get((m, tuple(tiling), variant)),
-            "kernel": ("a11_tile_pf_kernel<double, true>" if variant == 1
-                       else "a11_tile_kernel<double, true>"),
+            "kernel": kname,
             "kernel_ms": round(ms_kernel, 5),
             "alg_flops_per_launch": flops,
             "alg_bytes_per_launch": alg_bytes,
@@ -217,6 +256,7 @@ def main() -> int:
                    "ms_host_array_spmv_pcie": round(ms_host, 3),
                    "host_array_GDoF_s": round(ndof / (ms_host * 1e-3) / 1e9, 3),
                    "diagonal_GBs": round(8 * (6 * F.nnodes(m, m, m) + 27 * m ** 3) / (ms_diag * 1e-3) / 1e9, 1)},
+        "fp32_matmult": fp32,
     }
 
     if not args.no_cpu:
