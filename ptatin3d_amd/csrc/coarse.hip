@@ -6,6 +6,10 @@
 // (MatAssemble_StokesA_AUU, stokes_assembly.c:115-297, with
 // -fieldsplit_u_mg_coarse_pc_type lu, ptatin_driver_linear_ts.c:629-671).
 //
+// Nodes whose three dofs are all Dirichlet have identity rows and zero
+// columns: they are left out of every front (x = b there), which keeps the
+// factor to the free part of the operator.
+//
 // Ordering.  Nodes of the Q2 lattice couple only inside an element, so a
 // plane of nodes at an even index (an element boundary) separates the nodes
 // on its two sides.  The lattice box is bisected recursively by such planes
@@ -128,8 +132,14 @@ __global__ void k_mf_zero_upper(int nfr, int nf, int ns, double *__restrict__ G)
 // ---------------------------------------------------------------------------
 // kernels: solve
 // ---------------------------------------------------------------------------
-__global__ void k_mf_copy(long n, const double *__restrict__ x, double *__restrict__ y) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = x[i];
+// w = b and x = b (x keeps b on the dofs of all-Dirichlet nodes, which no
+// front holds; every other entry of x is written by the backward pass)
+__global__ void k_mf_copy(long n, const double *__restrict__ b, double *__restrict__ w, double *__restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double v = b[i];
+    w[i] = v;
+    x[i] = v;
+  }
 }
 
 // partial sums of [y_S; u_B] = G w_S: work-group (row block, column chunk,
@@ -370,7 +380,7 @@ long mf_factor_doubles(const MFFactor *m) { return m ? m->factor_doubles : 0; }
 double mf_setup_ms(const MFFactor *m) { return m ? m->setup_ms : 0.0; }
 int mf_depths(const MFFactor *m) { return m ? (int)m->groups.size() : 0; }
 
-int mf_factor(const Box &box, const double *S, hipStream_t stream, MFFactor **out) {
+int mf_factor(const Box &box, const double *S, const uint8_t *bc_host, hipStream_t stream, MFFactor **out) {
   *out = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   PTMF_HIP(hipEventCreate(&e0));
@@ -385,11 +395,15 @@ int mf_factor(const Box &box, const double *S, hipStream_t stream, MFFactor **ou
     nd_build(fr, lo, hi, 0, -1, 0);
   }
   int maxd = 0;
+  auto kept = [&](long n) { return !bc_host || (bc_host[n] & 7u) != 7u; };
   for (NDFront &F : fr) {
     maxd = std::max(maxd, F.depth);
     for (int k = F.slo[2]; k <= F.shi[2]; k++)
       for (int j = F.slo[1]; j <= F.shi[1]; j++)
-        for (int i = F.slo[0]; i <= F.shi[0]; i++) F.snodes.push_back(i + X * (j + Y * (long)k));
+        for (int i = F.slo[0]; i <= F.shi[0]; i++) {
+          const long n = i + X * (j + Y * (long)k);
+          if (kept(n)) F.snodes.push_back(n);
+        }
     int el[3], eh[3];
     for (int d = 0; d < 3; d++) {
       const long ext = d == 0 ? X : (d == 1 ? Y : Z);
@@ -401,7 +415,8 @@ int mf_factor(const Box &box, const double *S, hipStream_t stream, MFFactor **ou
         for (int i = el[0]; i <= eh[0]; i++) {
           const bool in = i >= F.lo[0] && i <= F.hi[0] && j >= F.lo[1] && j <= F.hi[1] && k >= F.lo[2] &&
                           k <= F.hi[2];
-          if (!in) F.bnodes.push_back(i + X * (j + Y * (long)k));
+          const long n = i + X * (j + Y * (long)k);
+          if (!in && kept(n)) F.bnodes.push_back(n);
         }
   }
   MFFactor *m = new MFFactor();
@@ -499,10 +514,14 @@ int mf_factor(const Box &box, const double *S, hipStream_t stream, MFFactor **ou
     if ((rc = upload(&g.sc_dof, sc_dof)) != PTMF_OK) break;
     if ((rc = upload(&g.sc_ptr, sc_ptr)) != PTMF_OK) break;
     if ((rc = upload(&g.sc_ent, sc_ent)) != PTMF_OK) break;
-    if ((rc = ptmf_malloc((void **)&g.G, sizeof(double) * (size_t)g.nfr * g.nf * g.ns)) != PTMF_OK) break;
-    if (g.nb > 0 && (rc = ptmf_malloc((void **)&g.ub, sizeof(double) * (size_t)g.nfr * g.nb)) != PTMF_OK) break;
-    if ((rc = ptmf_malloc((void **)&g.part, sizeof(double) * (size_t)g.nfr * g.ncc * g.nf)) != PTMF_OK) break;
-    if ((rc = ptmf_malloc((void **)&g.v, sizeof(double) * (size_t)g.nfr * g.nf)) != PTMF_OK) break;
+    // a depth whose fronts hold no free node (ns == 0) only passes its
+    // children's updates on: no solve factor, no solve work
+    if (g.ns > 0) {
+      if ((rc = ptmf_malloc((void **)&g.G, sizeof(double) * (size_t)g.nfr * g.nf * g.ns)) != PTMF_OK) break;
+      if (g.nb > 0 && (rc = ptmf_malloc((void **)&g.ub, sizeof(double) * (size_t)g.nfr * g.nb)) != PTMF_OK) break;
+      if ((rc = ptmf_malloc((void **)&g.part, sizeof(double) * (size_t)g.nfr * g.ncc * g.nf)) != PTMF_OK) break;
+      if ((rc = ptmf_malloc((void **)&g.v, sizeof(double) * (size_t)g.nfr * g.nf)) != PTMF_OK) break;
+    }
     m->factor_doubles += (long)g.nfr * g.nf * g.ns;
   }
   if (rc == PTMF_OK) rc = ptmf_malloc((void **)&m->w, sizeof(double) * m->n);
@@ -544,6 +563,7 @@ int mf_factor(const Box &box, const double *S, hipStream_t stream, MFFactor **ou
         goto done;
       }
       // L_SS = chol(F_SS)
+      if (g.ns > 0) {
       MF_BLAS(rocsolver_dpotrf_strided_batched(blas, rocblas_fill_lower, g.ns, g.F, g.nf, nff, dinfo, g.nfr));
       {
         std::vector<int> info(g.nfr);
@@ -583,6 +603,7 @@ int mf_factor(const Box &box, const double *S, hipStream_t stream, MFFactor **ou
         goto done;
       }
       k_mf_zero_upper<<<grid1((long)g.nfr * g.ns * g.ns), kBS, 0, stream>>>(g.nfr, g.nf, g.ns, g.G);
+      }  // ns > 0
       // extend-add into the parents (depth d - 1), one child slot at a time
       if (d > 0) {
         MFGroup &p = m->groups[d - 1];
@@ -631,9 +652,10 @@ done:
 // x = A^-1 b with the factor (kernels only: capturable, allocates nothing)
 int mf_solve(const MFFactor *m, const double *b, double *x, hipStream_t s) {
   const int nd = (int)m->groups.size();
-  k_mf_copy<<<grid1(m->n), kBS, 0, s>>>(m->n, b, m->w);
+  k_mf_copy<<<grid1(m->n), kBS, 0, s>>>(m->n, b, m->w, x);
   for (int d = nd - 1; d >= 0; d--) {
     const MFGroup &g = m->groups[d];
+    if (g.ns == 0) continue;
     const dim3 grid((unsigned)((g.nf + kFwdRows - 1) / kFwdRows), (unsigned)g.ncc, (unsigned)g.nfr);
     k_mf_fwd_part<<<grid, kFwdRows, 0, s>>>(g.nf, g.ns, g.ncc, g.G, g.sdof, m->w, g.part);
     k_mf_fwd_sum<<<grid1((long)g.nfr * g.nf), kBS, 0, s>>>(g.nfr, g.nf, g.ns, g.nb, g.ncc, g.part, g.sdof, m->w,
@@ -643,6 +665,7 @@ int mf_solve(const MFFactor *m, const double *b, double *x, hipStream_t s) {
   }
   for (int d = 0; d < nd; d++) {
     const MFGroup &g = m->groups[d];
+    if (g.ns == 0) continue;
     k_mf_bwd_gather<<<grid1((long)g.nfr * g.nf), kBS, 0, s>>>(g.nfr, g.nf, g.ns, g.nb, g.sdof, g.bdof, m->w, x,
                                                              g.v);
     k_mf_bwd<<<dim3((unsigned)((g.ns + 3) / 4), (unsigned)g.nfr), 256, 0, s>>>(g.nf, g.ns, g.G, g.sdof, g.v, x);

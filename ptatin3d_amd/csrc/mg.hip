@@ -1466,11 +1466,14 @@ int setup_coarse(ptmf_mg_s *mg) {
   (void)hipFree(mg->dense_part);
   mg->Adense = mg->dense_part = nullptr;
   mg->ndense = 0;
-  if (L.galerkin) return mf_factor(L.op->box, L.S64, mg->stream, &mg->mf);
+  // all-Dirichlet nodes (identity rows, zero columns) stay out of the factor
+  std::vector<uint8_t> bch(L.op->box.nnodes);
+  PTMF_HIP(hipMemcpy(bch.data(), L.op->bcnode, L.op->box.nnodes, hipMemcpyDeviceToHost));
+  if (L.galerkin) return mf_factor(L.op->box, L.S64, bch.data(), mg->stream, &mg->mf);
   double *S = nullptr;
   PTMF_TRY(dalloc((void **)&S, sizeof(double) * 1125 * L.op->box.nnodes));
   int st = probe_stencil(mg, 0, S);
-  if (st == PTMF_OK) st = mf_factor(L.op->box, S, mg->stream, &mg->mf);
+  if (st == PTMF_OK) st = mf_factor(L.op->box, S, bch.data(), mg->stream, &mg->mf);
   (void)hipStreamSynchronize(mg->stream);
   (void)hipFree(S);
   return st;

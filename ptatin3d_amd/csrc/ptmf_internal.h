@@ -191,7 +191,9 @@ int launch_a12(const ptmf_op_s *op, const double *p, double *yu);
 // unit Dirichlet rows), nested-dissection ordered (coarse.hip).  mf_solve is
 // kernels only (capturable, no allocation); b and x are device FP64 arrays.
 struct MFFactor;
-int mf_factor(const Box &box, const double *S, hipStream_t stream, MFFactor **out);
+// bc_host: per-node Dirichlet bits on the host (nodes with all three set are
+// left out of the factor) or null
+int mf_factor(const Box &box, const double *S, const uint8_t *bc_host, hipStream_t stream, MFFactor **out);
 int mf_solve(const MFFactor *m, const double *b, double *x, hipStream_t s);
 void mf_destroy(MFFactor *m);
 long mf_factor_doubles(const MFFactor *m);
