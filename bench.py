@@ -441,10 +441,25 @@ def batch_solve_leg(nbox: int) -> dict:
         E += m[0] * m[1] * m[2]
         ndof += b.size
     setup_s = time.perf_counter() - t0
+    # tile shape of every box's smoothed levels from the autotuner, in the
+    # V-cycle's precision (FP32); one tuning per distinct level box, reused
+    # by the boxes of the same shape (the batched MatMult runs variant 0)
+    from ptatin3d_amd import autotune
+    from ptatin3d_amd.operators import MatA11MF
+    t0 = time.perf_counter()
+    shapes = set()
+    for mg in mgs:
+        for lvl in (1, 2):
+            dims = mg.dims[lvl]
+            op = MatA11MF.borrow(mg.level_op_handle(lvl), *dims)
+            autotune.tune(op, precision=32, reps=3, max_shapes=3, variants=(0,), wg_tiles=(1,), persist=False)
+            shapes.add(tuple(dims))
+    tune_s = time.perf_counter() - t0
     bm = BatchGMG(mgs)
     res = {"boxes": nbox, "elements": E, "velocity_dofs": ndof, "levels": 3,
            "smoother": "FP32 Chebyshev(3)-Jacobi, FP64 dense coarse inverse, FP64 flexible CG to 1e-8",
-           "setup_seconds_all_boxes": round(setup_s, 2)}
+           "setup_seconds_all_boxes": round(setup_s, 2),
+           "tiling": f"autotuned per distinct level box in FP32 ({len(shapes)} boxes, {tune_s:.1f} s)"}
     for x in xs:
         x.zero()
     bm.fcg(bs, xs, rtol=1e-8, maxit=200)  # warm-up: captures the batched V-cycle
