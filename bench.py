@@ -495,6 +495,15 @@ def batch_solve_leg(nbox: int) -> dict:
                       "its_mean": round(float(np.mean(its1)), 2),
                       "max_rel_diff_vs_batched": diff}
     res["speedup_batched_vs_per_box"] = round(ts / tb, 2)
+    # error of the batched 1e-8 solutions against each box's FP64 solve
+    # continued to 1e-11 (true FP64 residuals, outside the timed regions)
+    errs = []
+    for mg, b, x, xbat in zip(mgs, bs, xs, xb):
+        x.upload(xbat)
+        mg.fcg(b, x, rtol=1e-11, maxit=400)
+        xt = x.download()
+        errs.append(float(np.linalg.norm(xbat - xt) / max(np.linalg.norm(xt), 1e-300)))
+    res["rel_err_vs_tight"] = {"max": max(errs), "mean": float(np.mean(errs))}
     for mg in mgs:
         mg.destroy()
     return res
