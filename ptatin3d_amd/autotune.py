@@ -88,12 +88,15 @@ def candidates(mx: int, my: int, mz: int, max_shapes: int = 8) -> list[tuple[int
 
 
 def tune(op, precision: int = 64, reps: int = 10, max_shapes: int = 8, verbose: bool = False,
-         variants=VARIANTS, persist: bool = True, wg_tiles=WG_TILES):
+         variants=VARIANTS, persist: bool = True, wg_tiles=WG_TILES, refine: int = 3):
     """Time the candidate (tiling, kernel variant, tiles per work-group)
     triples on `op` in the given precision and keep the fastest.  Variant 3
     (coloured scatter) is one candidate whatever the tiling; tiles per
-    work-group apply to variant 0.  Returns ((TX, TY, ZC), variant, tiles)
-    and the sorted timings (empty when the choice came from the cache)."""
+    work-group apply to variant 0.  The `refine` fastest of the first pass are
+    timed again (best of three runs of 2*reps MatMults each), so that
+    candidates a few per cent apart are not ranked by one noisy sample.
+    Returns ((TX, TY, ZC), variant, tiles) and the sorted timings (empty when
+    the choice came from the cache)."""
     _load()
     key = _key(op.mx, op.my, op.mz, precision, variants, max_shapes) + f":w{'-'.join(map(str, wg_tiles))}"
     if key in _CACHE:
@@ -126,6 +129,19 @@ def tune(op, precision: int = 64, reps: int = 10, max_shapes: int = 8, verbose: 
         if verbose:
             print(f"  tiling {t} variant {v} tiles/group {w}: {ms * 1e3:.1f} us", flush=True)
     results.sort()
+    if refine > 1 and len(results) > 1:
+        again = []
+        for _, (t, v, w) in results[:refine]:
+            op.set_kernel_variant(v)
+            op.set_tiling(*t)
+            op.set_tiles_per_group(w)
+            op.time_mult(x, y, 2)
+            again.append((min(op.time_mult(x, y, 2 * reps) for _ in range(3)), (t, v, w)))
+        again.sort()
+        results = again + results[refine:]
+        if verbose:
+            for ms, (t, v, w) in again:
+                print(f"  refined: tiling {t} variant {v} tiles/group {w}: {ms * 1e3:.1f} us", flush=True)
     best = results[0][1]
     op.set_tiling(*best[0])
     op.set_kernel_variant(best[1])
