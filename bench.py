@@ -54,7 +54,7 @@ FLOPS_PER_ELEMENT = 17253  # stokes_operators_tensor.c:281-283
 MEASURED_TRAFFIC = {
     (64, (4, 5, 2), 0): 269.6e6,
     (64, (7, 3, 2), 0): 275.0e6,
-    (64, (5, 4, 2), 0): 269.2e6,
+    (64, (5, 4, 2), 0): 270.2e6,
     (64, (3, 7, 2), 0): 276.6e6,
 }
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
