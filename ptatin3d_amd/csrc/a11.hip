@@ -276,11 +276,31 @@ __device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq
 #pragma unroll
     for (int p = 0; p < 9; p++) W[l][p] = T(0);
 
+  // zeta-plane values of the next point are read before this point's zeta
+  // part is written (point p writes slots p, 9+p, 18+p; point p+1 reads
+  // slots p+1, 9+p+1, ... only)
+  T Z[2][2][3][3];
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      Z[0][0][d][k] = sh[k][9 * d][le];
+      Z[0][1][d][k] = sh[k][27 + 9 * d][le];
+    }
 #pragma unroll
   for (int b = 0; b < 3; b++)
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       const int p = 3 * b + a;
+      if (p < 8) {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            Z[(p + 1) & 1][0][d][k] = sh[k][9 * d + p + 1][le];
+            Z[(p + 1) & 1][1][d][k] = sh[k][27 + 9 * d + p + 1][le];
+          }
+      }
       const T etq = etq9[p];
       // M[j][d] = d x_d / d xi_j, Gu[j][l] = d u_l / d xi_j
       T M[3][3], Gu[3][3];
@@ -288,12 +308,10 @@ __device__ __forceinline__ void plane_qp(int cp, int le, bool writer, const T Xq
       for (int d = 0; d < 3; d++) {
         M[0][d] = cdot<T>(a, Xq[d][3 * b], Xq[d][3 * b + 1], Xq[d][3 * b + 2]);
         M[1][d] = cdot<T>(b, Xq[d][a], Xq[d][3 + a], Xq[d][6 + a]);
-        M[2][d] = dz[0] * sh[0][9 * d + p][le] + dz[1] * sh[1][9 * d + p][le] +
-                  dz[2] * sh[2][9 * d + p][le];
+        M[2][d] = dz[0] * Z[p & 1][0][d][0] + dz[1] * Z[p & 1][0][d][1] + dz[2] * Z[p & 1][0][d][2];
         Gu[0][d] = cdot<T>(a, Uq[d][3 * b], Uq[d][3 * b + 1], Uq[d][3 * b + 2]);
         Gu[1][d] = cdot<T>(b, Uq[d][a], Uq[d][3 + a], Uq[d][6 + a]);
-        Gu[2][d] = dz[0] * sh[0][27 + 9 * d + p][le] + dz[1] * sh[1][27 + 9 * d + p][le] +
-                   dz[2] * sh[2][27 + 9 * d + p][le];
+        Gu[2][d] = dz[0] * Z[p & 1][1][d][0] + dz[1] * Z[p & 1][1][d][1] + dz[2] * Z[p & 1][1][d][2];
       }
       // adjugate A = det * M^-1 (A[k][j] = det * d xi_j / d x_k)
       T A[3][3];
