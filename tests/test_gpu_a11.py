@@ -144,6 +144,8 @@ def test_config2_parity_and_properties(gpu, cfg2):
     # repeat apply: the tiled scatter has a fixed summation order
     A.mult(dx, dy)
     assert np.array_equal(dy.download(), y)
+    # the FP32 MatMult of the V-cycle levels on the same workload
+    assert rel_inf(A.mult_host(x, precision=32), yo) <= TOL32
     # symmetry on the free space
     z = np.random.default_rng(9).uniform(-1, 1, x.size)
     z[c["bc"].astype(bool)] = 0.0
