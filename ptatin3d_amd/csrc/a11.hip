@@ -634,8 +634,12 @@ __global__ void __launch_bounds__(256)
 }
 
 // Write the completed node plane P of the window: owned nodes to y, the
-// tile's upper-face nodes to the face buffers (x face first, then y, then z).
+// tile's upper-face nodes to the face buffers (the x face column, corner
+// included, then the y face row, then on the tile's top plane the z face).
 // Pure LDS reads and global stores; clears the window entries it reads.
+// Three loops, one destination each (no divergent stores): the owned block
+// (node columns il < nxo, rows jl < nyo), the upper x face column and the
+// upper y face row.
 template <typename T, int kWaves = 1>
 __device__ __forceinline__ void flush_plane(const TileArgs &a, long tile, const TileBox &tb, int P,
                                             T *__restrict__ win, T *__restrict__ y,
@@ -643,28 +647,49 @@ __device__ __forceinline__ void flush_plane(const TileArgs &a, long tile, const 
                                             T *__restrict__ fbz) {
   const int wX = 2 * tb.nxt + 1, wY = 2 * tb.nyt + 1;
   const int W3 = 3 * wX;
-  const float inv = 1.0f / (float)W3;
   const int kl = P - 2 * tb.z0;
-  const bool oz = (kl < 2 * tb.nzt) || tb.zend;
-  const long rowbase = (long)(2 * tb.x0) + a.XY * (long)P;
-  for (int t = wave_lane<kWaves>(); t < W3 * wY; t += 64) {
-    int jl = (int)((float)t * inv);
-    if (jl * W3 > t) jl--;
-    if ((jl + 1) * W3 <= t) jl++;
-    const int r = t - jl * W3;
-    const int il = r / 3, l = r - 3 * il;
-    const T v = win[t];
-    win[t] = T(0);
-    const bool ox = (il < 2 * tb.nxt) || tb.xend;
-    const bool oy = (jl < 2 * tb.nyt) || tb.yend;
-    if (!ox) {
-      fbx[3 * (tile * a.fbx_n + jl + (long)(2 * a.TY + 1) * kl) + l] = v;
-    } else if (!oy) {
-      fby[3 * (tile * a.fby_n + il + (long)(2 * a.TX + 1) * kl) + l] = v;
-    } else if (!oz) {
-      fbz[3 * (tile * a.fbz_n + il + (long)(2 * a.TX + 1) * jl) + l] = v;
-    } else {
-      y[3 * (rowbase + il + a.X * (long)(2 * tb.y0 + jl)) + l] = v;
+  const int nxo = tb.xend ? wX : 2 * tb.nxt;  // owned node columns
+  const int nyo = tb.yend ? wY : 2 * tb.nyt;  // owned node rows
+  const int R3 = 3 * nxo;
+  const int lane = wave_lane<kWaves>();
+  {
+    // owned block: entry t = jl * R3 + r (r = 3 il + l) is window entry jl * W3 + r
+    const bool zface = kl >= 2 * tb.nzt && !tb.zend;
+    T *const dst = zface ? fbz : y;
+    const long off = zface ? 3 * tile * a.fbz_n
+                           : 3 * ((long)(2 * tb.x0) + a.XY * (long)P + a.X * (long)(2 * tb.y0));
+    const long rs = zface ? 3L * (2 * a.TX + 1) : 3 * a.X;
+    const float inv = 1.0f / (float)R3;
+    for (int t = lane; t < R3 * nyo; t += 64) {
+      int jl = (int)((float)t * inv);
+      if (jl * R3 > t) jl--;
+      if ((jl + 1) * R3 <= t) jl++;
+      const int r = t - jl * R3;
+      const int w = jl * W3 + r;
+      const T v = win[w];
+      win[w] = T(0);
+      dst[off + rs * jl + r] = v;
+    }
+  }
+  if (!tb.xend) {
+    // upper x face: node column il = 2 nxt of every row, t = 3 jl + l
+    const long off = 3 * (tile * a.fbx_n + (long)(2 * a.TY + 1) * kl);
+    for (int t = lane; t < 3 * wY; t += 64) {
+      const int jl = t / 3;
+      const int w = jl * W3 + 6 * tb.nxt + (t - 3 * jl);
+      const T v = win[w];
+      win[w] = T(0);
+      fbx[off + t] = v;
+    }
+  }
+  if (!tb.yend) {
+    // upper y face: node row jl = 2 nyt without the x face corner, t = 3 il + l
+    const long off = 3 * (tile * a.fby_n + (long)(2 * a.TX + 1) * kl);
+    for (int t = lane; t < R3; t += 64) {
+      const int w = 2 * tb.nyt * W3 + t;
+      const T v = win[w];
+      win[w] = T(0);
+      fby[off + t] = v;
     }
   }
 }
