@@ -735,26 +735,33 @@ __device__ __forceinline__ void tile_march(const TileArgs &a, long tile, const T
       element_plane_apply<T, kMask, kWaves>(cp, lc, wr, eta + tq_index(tile, a.ZC, kz, 9 * cp, le), kEPW, base,
                                             a.X, xc, bcn, u, sh, out);
     }
-    // Sum into the window.  For a fixed node slot (i,j) no two lanes touch
-    // the same window entry, so the nine slots are added one after another
-    // with plain read-add-write; the wave executes LDS operations in order
-    // and the wave barriers keep the compiler from interleaving the steps.
+    // Sum into the window with plain read-add-write.  Element (lx, ly)
+    // touches nodes (2lx + i, 2ly + j); only slots i = 2 / j = 2 meet the
+    // slots i = 0 / j = 0 of the next element, so the nine slots go in four
+    // groups, within which no two lanes touch the same entry: {i, j < 2},
+    // {i = 2, j < 2}, {i < 2, j = 2}, {(2, 2)}.  The wave executes LDS
+    // operations in order; the barriers between groups keep the compiler
+    // from interleaving them.
     {
       T *w = win[(2 * kz + cp) % 3] + 3 * ((2 * ly) * wX + 2 * lx);
 #pragma unroll
-      for (int j = 0; j < 3; j++)
+      for (int g = 0; g < 4; g++) {
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-          if (live) {
-            T *q = w + 3 * (j * wX + i);
-            const T q0 = q[0], q1 = q[1], q2 = q[2];
-            q[0] = q0 + out[0][3 * j + i];
-            q[1] = q1 + out[1][3 * j + i];
-            q[2] = q2 + out[2][3 * j + i];
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            if ((i == 2 ? 1 : 0) + (j == 2 ? 2 : 0) != g) continue;
+            if (live) {
+              T *q = w + 3 * (j * wX + i);
+              const T q0 = q[0], q1 = q[1], q2 = q[2];
+              q[0] = q0 + out[0][3 * j + i];
+              q[1] = q1 + out[1][3 * j + i];
+              q[2] = q2 + out[2][3 * j + i];
+            }
           }
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
     }
     tsync<kWaves>();
     flush_plane<T, kWaves>(a, tile, tb, (int)(2 * ek), win[(2 * kz) % 3], y, fbx, fby, fbz);
