@@ -1467,23 +1467,33 @@ int launch_a11_mult(ptmf_op_s *op, const T *x, T *y, int faces) {
   return launch_a11_mult_fb<T>(op, x, y, faces, (T *)op->faces);
 }
 
-int launch_stokes_mult(ptmf_op_s *op, const double *x, double *y) {
+template <typename T>
+int launch_stokes_mult_t(ptmf_op_s *op, const T *x, T *y) {
+  const OpData<T> &d = *(const OpData<T> *)(sizeof(T) == 8 ? (const void *)&op->d64
+                                                           : (const void *)&op->d32);
   const Tiling &tl = op->tiling;
   PTMF_CHECK(tl.TX * tl.TY <= kEPW && (2 * tl.TX + 1) * (2 * tl.TY + 1) <= kWinNodes && tl.ZC > 0,
              PTMF_ERR_ARG, "stokes: invalid tiling %d x %d x %d", tl.TX, tl.TY, tl.ZC);
   PTMF_CHECK(op->faces != nullptr, PTMF_ERR_NOT_SETUP, "stokes: face buffers not allocated");
-  PTMF_TRY(ensure_eta_tiled<double>(op));
+  PTMF_TRY(ensure_eta_tiled<T>(op));
+  const T *et = sizeof(T) == 8 ? (const T *)op->et64 : (const T *)op->et32;
   const TileArgs a = tile_args(op);
   const long ntiles = tl.ntiles, nu = 3 * op->box.nnodes;
-  double *fbx = (double *)op->faces, *fby = fbx + 3 * a.fbx_n * ntiles, *fbz = fby + 3 * a.fby_n * ntiles;
-  stokes_tile_kernel<double><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
-      a, op->d64.coords, op->et64, op->bcnode, x, x + nu, y, y + nu, fbx, fby, fbz);
+  T *fbx = (T *)op->faces, *fby = fbx + 3 * a.fbx_n * ntiles, *fbz = fby + 3 * a.fby_n * ntiles;
+  stokes_tile_kernel<T><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
+      a, d.coords, et, op->bcnode, x, x + nu, y, y + nu, fbx, fby, fbz);
   PTMF_HIP(hipGetLastError());
-  a11_face_kernel<double><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(a, fbx, fby, fbz, y);
+  a11_face_kernel<T><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(a, fbx, fby, fbz, y);
   PTMF_HIP(hipGetLastError());
-  PTMF_TRY(launch_bc_copy<double>(op->bcidx, op->nbc, x, y, op->stream));
+  PTMF_TRY(launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
   return PTMF_OK;
 }
+
+int launch_stokes_mult(ptmf_op_s *op, const double *x, double *y) { return launch_stokes_mult_t<double>(op, x, y); }
+
+// FP32 saddle-point MatMult (FP32 coordinates, viscosity and accumulation):
+// the preconditioner's A12 product of an FP32 velocity V-cycle.
+int launch_stokes_mult_f32(ptmf_op_s *op, const float *x, float *y) { return launch_stokes_mult_t<float>(op, x, y); }
 
 int prepare_eta_tiled(ptmf_op_s *op) { return ensure_eta_tiled<double>(op); }
 

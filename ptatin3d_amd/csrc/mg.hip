@@ -61,6 +61,13 @@ __global__ void k_residual(long n, const T *__restrict__ b, const T *Ax, T *r) {
     r[i] = b[i] - Ax[i];
 }
 
+// r = b - Ax with an FP32 product
+__global__ void k_residual_mixed(long n, const double *__restrict__ b, const float *__restrict__ Ax,
+                                 double *__restrict__ r) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    r[i] = b[i] - (double)Ax[i];
+}
+
 template <typename Ts, typename Td>
 __global__ void k_cast(long n, const Ts *__restrict__ x, Td *__restrict__ y) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -943,6 +950,8 @@ struct ptmf_mg_s {
   // saddle point: inverse Schur blocks (16 per element) and a velocity temporary
   double *sMinv = nullptr, *stmp = nullptr;
   double *sx2 = nullptr, *sy2 = nullptr;  // [u | p] temporaries of the preconditioner's A12 product
+  float *sx2f = nullptr, *sy2f = nullptr;  // the same in FP32 (FP32 velocity V-cycle)
+  double *kvb = nullptr, *kvx = nullptr;   // fixed V-cycle input/output of the FGMRES preconditioners
   bool stokes_ready = false;
   // captured V-cycles (levels_mg == L), one per (b, x) pair of device
   // pointers the cycle was called with: a replay reads b and writes x
@@ -1694,6 +1703,25 @@ int stokes_apply(ptmf_mg_s *mg, const double *x, double *y) {
   return launch_stokes_mult(mg->lev[mg->L - 1].op, x, y);
 }
 
+// One V-cycle from b into z for callers whose vectors change from call to
+// call (FGMRES hands the preconditioner a new basis vector every iteration):
+// the cycle runs between two buffers of the hierarchy's own, b copied in
+// unless it is already the fixed input and the result copied out, so one
+// captured graph serves every iteration (a graph is keyed by its pointers).
+int vcycle_fixed(ptmf_mg_s *mg, const double *b, double *z) {
+  const long n = mg->lev[mg->L - 1].n;
+  if (!mg->kvb) PTMF_TRY(dalloc((void **)&mg->kvb, sizeof(double) * n));
+  if (!mg->kvx) PTMF_TRY(dalloc((void **)&mg->kvx, sizeof(double) * n));
+  const double *in = b;
+  if (b != mg->stmp) {
+    PTMF_HIP(hipMemcpyAsync(mg->kvb, b, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+    in = mg->kvb;
+  }
+  PTMF_TRY(ptmf_mg_vcycle(mg, in, mg->kvx));
+  PTMF_HIP(hipMemcpyAsync(z, mg->kvx, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+  return PTMF_OK;
+}
+
 // Upper block-triangular preconditioner [A11 A12; 0 S]^-1 with S the scaled
 // viscous pressure mass (block diagonal) and A11^-1 one V-cycle:
 // zp = S^-1 rp, zu = V-cycle(ru - A12 zp)  (the fieldsplit Schur "upper"
@@ -1704,13 +1732,24 @@ int stokes_prec(ptmf_mg_s *mg, const double *r, double *z) {
   const long np = 4 * F.op->box.nel;
   PTMF_TRY(launch_schur_apply(F.op, mg->sMinv, r + nu, z + nu));
   // stmp = r_u - A12 z_p, A12 z_p from the fused saddle-point kernel on
-  // [0 | z_p] (one element pass; its Dirichlet rows come out as 0)
-  PTMF_TRY(launch_fill<double>(mg->sx2, 0.0, nu, mg->stream));
-  PTMF_HIP(hipMemcpyAsync(mg->sx2 + nu, z + nu, sizeof(double) * np, hipMemcpyDeviceToDevice, mg->stream));
-  PTMF_TRY(launch_stokes_mult(F.op, mg->sx2, mg->sy2));
-  k_residual<double><<<nblocks(nu), kBS, 0, mg->stream>>>(nu, r, mg->sy2, mg->stmp);
+  // [0 | z_p] (one element pass; its Dirichlet rows come out as 0).  With an
+  // FP32 velocity V-cycle the product is formed in FP32 like the rest of the
+  // velocity preconditioner (its result only feeds that V-cycle); z_p, the
+  // Schur update, stays FP64.
+  if (mg->opt.smoother_precision == 32 && mg->sx2f) {
+    PTMF_TRY(launch_fill<float>(mg->sx2f, 0.0f, nu, mg->stream));
+    k_cast<double, float><<<nblocks(np), kBS, 0, mg->stream>>>(np, z + nu, mg->sx2f + nu);
+    PTMF_HIP(hipGetLastError());
+    PTMF_TRY(launch_stokes_mult_f32(F.op, mg->sx2f, mg->sy2f));
+    k_residual_mixed<<<nblocks(nu), kBS, 0, mg->stream>>>(nu, r, mg->sy2f, mg->stmp);
+  } else {
+    PTMF_TRY(launch_fill<double>(mg->sx2, 0.0, nu, mg->stream));
+    PTMF_HIP(hipMemcpyAsync(mg->sx2 + nu, z + nu, sizeof(double) * np, hipMemcpyDeviceToDevice, mg->stream));
+    PTMF_TRY(launch_stokes_mult(F.op, mg->sx2, mg->sy2));
+    k_residual<double><<<nblocks(nu), kBS, 0, mg->stream>>>(nu, r, mg->sy2, mg->stmp);
+  }
   PTMF_HIP(hipGetLastError());
-  if (mg->s_inner_its <= 1) return ptmf_mg_vcycle(mg, mg->stmp, z);
+  if (mg->s_inner_its <= 1) return vcycle_fixed(mg, mg->stmp, z);
   PTMF_HIP(hipMemsetAsync(z, 0, sizeof(double) * nu, mg->stream));
   int its = 0;
   double rn = 0.0;
@@ -1827,6 +1866,10 @@ int ptmf_mg_destroy(ptmf_mg *pmg) {
   (void)hipFree(mg->stmp);
   (void)hipFree(mg->sx2);
   (void)hipFree(mg->sy2);
+  (void)hipFree(mg->sx2f);
+  (void)hipFree(mg->sy2f);
+  (void)hipFree(mg->kvb);
+  (void)hipFree(mg->kvx);
   ptmf::drop_graph(mg);
   for (double *p : mg->pcg) (void)hipFree(p);
   for (double *p : mg->fcg) (void)hipFree(p);
@@ -2198,7 +2241,7 @@ int ptmf_mg_fgmres(ptmf_mg mg, const double *b, double *x, double rtol, int maxi
   PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_fgmres: not set up");
   MGLevel &F = mg->lev[mg->L - 1];
   auto apply = [&](const double *v, double *y) { return ptmf::matmult<double>(F, v, y); };
-  auto prec = [&](const double *r, double *z) { return ptmf_mg_vcycle(mg, r, z); };
+  auto prec = [&](const double *r, double *z) { return ptmf::vcycle_fixed(mg, r, z); };
   return ptmf::fgmres_core(mg, mg->ku, F.n, apply, prec, b, x, rtol, maxit, restart, its_out, rnorm_out,
                            history, history_len);
 }
@@ -2293,6 +2336,10 @@ int ptmf_mg_stokes_setup(ptmf_mg mg, int inner_its, double inner_rtol) {
   const long nup = F.n + 4 * F.op->box.nel;
   if (!mg->sx2) PTMF_TRY(ptmf::dalloc((void **)&mg->sx2, sizeof(double) * nup));
   if (!mg->sy2) PTMF_TRY(ptmf::dalloc((void **)&mg->sy2, sizeof(double) * nup));
+  if (mg->opt.smoother_precision == 32) {
+    if (!mg->sx2f) PTMF_TRY(ptmf::dalloc((void **)&mg->sx2f, sizeof(float) * nup));
+    if (!mg->sy2f) PTMF_TRY(ptmf::dalloc((void **)&mg->sy2f, sizeof(float) * nup));
+  }
   PTMF_TRY(ptmf::launch_schur_blocks(F.op, nullptr, mg->sMinv));
   PTMF_HIP(hipStreamSynchronize(mg->stream));
   mg->stokes_ready = true;

@@ -169,6 +169,7 @@ int launch_bc_set(const long *idx, long n, T v, T *y, hipStream_t s);
 int launch_copy16(const void *x, void *y, long n16, hipStream_t s);
 int launch_eta_qpmajor(long nel, const double *em, double *q64, float *q32, hipStream_t s);
 int launch_stokes_mult(ptmf_op_s *op, const double *x, double *y);
+int launch_stokes_mult_f32(ptmf_op_s *op, const float *x, float *y);
 // element blocks (-B_e^T D^-1 B_e)^-1; *singular (device, zeroed by the
 // caller) is set when a block has fewer than 4 independent free rows
 int launch_schur_bdb(const ptmf_op_s *op, const double *dg, double *Minv, int *singular);
