@@ -617,7 +617,8 @@ def stokes_leg(dims, maxit: int, schur: str = "mass", restart: int = 40) -> dict
            "krylov": f"FGMRES({restart}), FP64, fused saddle-point MatMult",
            "preconditioner": "upper block-triangular: " + {"mass": "scaled viscous-mass",
                                                             "bdb": "B^T diag(A11)^-1 B element-block"}[schur]
-                             + " Schur, one V-cycle for A11"}
+                             + " Schur (FP64), one V-cycle for A11; with the FP32 V-cycle the A12 z_p product "
+                               "that forms the V-cycle's right-hand side is FP32 too"}
     if schur == "bdb":
         res["deviation"] = ("Schur approximation: element blocks of B^T diag(A11)^-1 B (no counterpart in the "
                             "reference) instead of the reference's scaled viscous mass, with which FGMRES stalls "
