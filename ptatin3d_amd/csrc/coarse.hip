@@ -50,7 +50,8 @@ namespace {
 
 constexpr int kLeafNodes = 64;
 constexpr int kFwdRows = 256;   // rows per work-group of the forward product
-constexpr int kFwdCols = 256;   // columns per chunk of the forward product
+constexpr int kFwdCols = 256;   // largest column chunk of the forward product
+constexpr long kFwdMinGroups = 2048;  // work-groups a depth's forward product aims for
 constexpr int kBS = 256;
 
 inline unsigned grid1(long n) {
@@ -142,15 +143,17 @@ __global__ void k_mf_copy(long n, const double *__restrict__ b, double *__restri
   }
 }
 
-// partial sums of [y_S; u_B] = G w_S: work-group (row block, column chunk,
-// front); rows of the triangular top block stop at the diagonal
+// partial sums of [y_S; u_B] = G w_S: work-group (row block, column chunk
+// of cw <= kFwdCols columns, front); rows of the triangular top block stop at
+// the diagonal.  Every (chunk, row) partial is written, zero where the chunk
+// lies right of the row's diagonal.
 __global__ void __launch_bounds__(kFwdRows)
-    k_mf_fwd_part(int nf, int ns, int ncc, const double *__restrict__ G, const long *__restrict__ sdof,
+    k_mf_fwd_part(int nf, int ns, int ncc, int cw, const double *__restrict__ G, const long *__restrict__ sdof,
                   const double *__restrict__ w, double *__restrict__ part) {
   __shared__ double xs[kFwdCols];
   const long f = blockIdx.z;
-  const int j0 = blockIdx.y * kFwdCols;
-  const int jn = min(kFwdCols, ns - j0);
+  const int j0 = blockIdx.y * cw;
+  const int jn = min(cw, ns - j0);
   const long *sd = sdof + f * ns;
   for (int t = threadIdx.x; t < jn; t += kFwdRows) {
     const long g = sd[j0 + t];
@@ -162,14 +165,16 @@ __global__ void __launch_bounds__(kFwdRows)
   int je = jn;
   if (i < ns) je = min(jn, i - j0 + 1);  // G[i, j] = 0 for j > i in the top block
   const double *a = G + f * (long)nf * ns + i + (long)j0 * nf;
-  double s0 = 0.0, s1 = 0.0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   int j = 0;
-  for (; j + 2 <= je; j += 2) {
+  for (; j + 4 <= je; j += 4) {
     s0 += a[(long)j * nf] * xs[j];
     s1 += a[(long)(j + 1) * nf] * xs[j + 1];
+    s2 += a[(long)(j + 2) * nf] * xs[j + 2];
+    s3 += a[(long)(j + 3) * nf] * xs[j + 3];
   }
-  if (j < je) s0 += a[(long)j * nf] * xs[j];
-  part[(f * ncc + blockIdx.y) * (long)nf + i] = s0 + s1;
+  for (; j < je; j++) s0 += a[(long)j * nf] * xs[j];
+  part[(f * ncc + blockIdx.y) * (long)nf + i] = (s0 + s1) + (s2 + s3);
 }
 
 // y_S into w, u_B into the front's boundary buffer
@@ -218,8 +223,8 @@ __global__ void k_mf_bwd_gather(int nfr, int nf, int ns, int nb, const long *__r
   }
 }
 
-// x_S = G^T v: one wave per column (rows from the diagonal down), four
-// columns per work-group
+// x_S = G^T v: one wave per column (rows from the diagonal down, four
+// independent partial sums per lane), four columns per work-group
 __global__ void __launch_bounds__(256)
     k_mf_bwd(int nf, int ns, const double *__restrict__ G, const long *__restrict__ sdof,
              const double *__restrict__ v, double *__restrict__ x) {
@@ -229,8 +234,16 @@ __global__ void __launch_bounds__(256)
   if (j >= ns) return;
   const double *g = G + f * (long)nf * ns + (long)j * nf;
   const double *vf = v + f * (long)nf;
-  double s = 0.0;
-  for (int i = j + lane; i < nf; i += 64) s += g[i] * vf[i];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = j + lane;
+  for (; i + 192 < nf; i += 256) {
+    s0 += g[i] * vf[i];
+    s1 += g[i + 64] * vf[i + 64];
+    s2 += g[i + 128] * vf[i + 128];
+    s3 += g[i + 192] * vf[i + 192];
+  }
+  for (; i < nf; i += 64) s0 += g[i] * vf[i];
+  double s = (s0 + s1) + (s2 + s3);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) {
     const long gd = sdof[f * ns + j];
@@ -312,7 +325,7 @@ int nd_build(std::vector<NDFront> &fr, const int lo[3], const int hi[3], int dep
 
 // One depth of the tree: its fronts padded to common sizes.
 struct MFGroup {
-  int nfr = 0, ns = 0, nb = 0, nf = 0, ncc = 0;
+  int nfr = 0, ns = 0, nb = 0, nf = 0, ncc = 0, cw = kFwdCols;
   std::vector<int> ids;  // front ids of this depth
   double *F = nullptr;   // nfr x nf x nf, set-up only
   double *G = nullptr;   // nfr x nf x ns
@@ -436,7 +449,12 @@ int mf_factor(const Box &box, const double *S, const uint8_t *bc_host, hipStream
   for (MFGroup &g : m->groups) {
     g.nfr = (int)g.ids.size();
     g.nf = g.ns + g.nb;
-    g.ncc = (g.ns + kFwdCols - 1) / kFwdCols;
+    // narrower column chunks where a depth has few or small fronts, so the
+    // forward product has enough work-groups to stream its factor block
+    g.cw = kFwdCols;
+    const long rows = (long)g.nfr * ((g.nf + kFwdRows - 1) / kFwdRows);
+    while (g.cw > 32 && rows * ((g.ns + g.cw - 1) / g.cw) < kFwdMinGroups) g.cw /= 2;
+    g.ncc = (g.ns + g.cw - 1) / g.cw;
   }
   // ---- index tables --------------------------------------------------------
   // local dof base of every node of a front: S node p -> 3p, B node q -> ns + 3q
@@ -657,7 +675,7 @@ int mf_solve(const MFFactor *m, const double *b, double *x, hipStream_t s) {
     const MFGroup &g = m->groups[d];
     if (g.ns == 0) continue;
     const dim3 grid((unsigned)((g.nf + kFwdRows - 1) / kFwdRows), (unsigned)g.ncc, (unsigned)g.nfr);
-    k_mf_fwd_part<<<grid, kFwdRows, 0, s>>>(g.nf, g.ns, g.ncc, g.G, g.sdof, m->w, g.part);
+    k_mf_fwd_part<<<grid, kFwdRows, 0, s>>>(g.nf, g.ns, g.ncc, g.cw, g.G, g.sdof, m->w, g.part);
     k_mf_fwd_sum<<<grid1((long)g.nfr * g.nf), kBS, 0, s>>>(g.nfr, g.nf, g.ns, g.nb, g.ncc, g.part, g.sdof, m->w,
                                                           g.ub);
     if (g.nscat > 0)
