@@ -1147,7 +1147,8 @@ __global__ void __launch_bounds__(64)
 }
 
 // Adds the neighbours' face-buffer contributions into the nodes on the lower
-// faces of each tile.
+// faces of each tile (the work-group's threads take one node each in turn;
+// every node's sum has one thread and a fixed order).
 template <typename T>
 __device__ __forceinline__ void face_sum(const TileArgs &a, long tile, const T *__restrict__ fbx,
                                          const T *__restrict__ fby, const T *__restrict__ fbz,
@@ -1161,7 +1162,7 @@ __device__ __forceinline__ void face_sum(const TileArgs &a, long tile, const T *
   const int iy0 = hx ? 1 : 0, jz0 = hy ? 1 : 0, iz0 = hx ? 1 : 0;
   const int nY = hy ? (oX - iy0) * oZ : 0;
   const int nZ = hz ? (oX - iz0) * (oY - jz0) : 0;
-  for (int t = threadIdx.x; t < nX + nY + nZ; t += 64) {
+  for (int t = threadIdx.x; t < nX + nY + nZ; t += blockDim.x) {
     int il, jl, kl;
     if (t < nX) {
       il = 0; jl = t % oY; kl = t / oY;
@@ -1198,8 +1199,13 @@ __device__ __forceinline__ void face_sum(const TileArgs &a, long tile, const T *
   }
 }
 
+// One tile per work-group of kFaceThreads, one face node per thread: the
+// tall tiles of long z-chunks have 400-800 lower-face nodes, which a single
+// wave would walk one dependent load/store round after another.
+constexpr int kFaceThreads = 256;
+
 template <typename T>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kFaceThreads)
     a11_face_kernel(TileArgs a, const T *__restrict__ fbx, const T *__restrict__ fby,
                     const T *__restrict__ fbz, T *__restrict__ y) {
   face_sum<T>(a, xcd_tile(blockIdx.x, a.ntiles), fbx, fby, fbz, y);
@@ -1455,7 +1461,7 @@ int launch_a11_mult_fb(ptmf_op_s *op, const T *x, T *y, int faces, T *fb, bool n
   }
   PTMF_HIP(hipGetLastError());
   if (faces) {
-    a11_face_kernel<T><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(a, fbx, fby, fbz, y);
+    a11_face_kernel<T><<<dim3((unsigned)ntiles), dim3(kFaceThreads), 0, op->stream>>>(a, fbx, fby, fbz, y);
     PTMF_HIP(hipGetLastError());
     if (faces == 1) PTMF_TRY(launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
   }
@@ -1483,7 +1489,7 @@ int launch_stokes_mult_t(ptmf_op_s *op, const T *x, T *y) {
   stokes_tile_kernel<T><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(
       a, d.coords, et, op->bcnode, x, x + nu, y, y + nu, fbx, fby, fbz);
   PTMF_HIP(hipGetLastError());
-  a11_face_kernel<T><<<dim3((unsigned)ntiles), dim3(64), 0, op->stream>>>(a, fbx, fby, fbz, y);
+  a11_face_kernel<T><<<dim3((unsigned)ntiles), dim3(kFaceThreads), 0, op->stream>>>(a, fbx, fby, fbz, y);
   PTMF_HIP(hipGetLastError());
   PTMF_TRY(launch_bc_copy<T>(op->bcidx, op->nbc, x, y, op->stream));
   return PTMF_OK;
