@@ -282,6 +282,26 @@ int ptmf_mg_vcycle(ptmf_mg mg, const double *b, double *x);
 /* y = A_level x with the level's operator (rediscretised matrix-free or
  * Galerkin stencil), precision 64 (double*) or 32 (float*) */
 int ptmf_mg_level_apply(ptmf_mg mg, int level, int precision, const void *x, void *y);
+/* The level's operator assembled as a CSR matrix on the host
+ * (MatAssemble_StokesA_AUU, stokes_assembly.c:115-297, with the Dirichlet
+ * treatment of BCListInsertScaling, dmda_bcs.c:1035-1060: Dirichlet rows and
+ * columns removed, unit diagonal inserted), as the reference builds its
+ * assembled levels (ptatin_driver_linear_ts.c:640-662).  Rows and columns in
+ * the node-interlaced dof order 3n + c, 0-based; a free row holds the
+ * couplings to the free dofs of every node sharing an element with its node,
+ * columns ascending.  Values from the level's stencil (Galerkin levels) or its
+ * MatMult probed with 375 colourings.  First call with colidx = vals = NULL:
+ * *nnz, and rowptr (3N + 1 entries) when not NULL; then with colidx and vals
+ * of *nnz entries and the same rowptr.  Needs ptmf_mg_setup; host work and
+ * PCIe traffic grow as 1125 N doubles, meant for coarse levels. */
+int ptmf_mg_level_assemble(ptmf_mg mg, int level, long *nnz, long *rowptr, long *colidx, double *vals);
+/* The six rigid-body modes of a level from its node coordinates, the near
+ * null space the reference attaches to its assembled levels
+ * (ptatin_driver_linear_ts.c:663-666, PETSc's MatNullSpaceCreateRigidBody):
+ * modes[k * 3N + 3n + c], k = 0..2 translations along x, y, z, k = 3 rotation
+ * about z (-y, x, 0), k = 4 about x (0, -z, y), k = 5 about y (z, 0, -x).
+ * Host array of 6 * 3N doubles, not normalised. */
+int ptmf_mg_level_rigid_body_modes(ptmf_mg mg, int level, double *modes);
 /* yf += P xc and yc = P^T xf between levels level-1 and level (device FP64) */
 int ptmf_mg_prolong_add(ptmf_mg mg, int level, const double *xc, double *yf);
 int ptmf_mg_restrict(ptmf_mg mg, int level, const double *xf, double *yc);

@@ -85,6 +85,8 @@ SIGNATURES: dict[str, tuple] = {
     "ptmf_mg_set_level_eigs": (_I, [_P, _I, _D, _D]),
     "ptmf_mg_vcycle": (_I, [_P, _P, _P]),
     "ptmf_mg_level_apply": (_I, [_P, _I, _I, _P, _P]),
+    "ptmf_mg_level_assemble": (_I, [_P, _I, C.POINTER(_L), _P, _P, _P]),
+    "ptmf_mg_level_rigid_body_modes": (_I, [_P, _I, _P]),
     "ptmf_mg_prolong_add": (_I, [_P, _I, _P, _P]),
     "ptmf_mg_restrict": (_I, [_P, _I, _P, _P]),
     "ptmf_mg_smooth": (_I, [_P, _I, _I, _I, _I, _P, _P]),

@@ -22,6 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -2202,6 +2203,135 @@ int ptmf_mg_level_apply(ptmf_mg mg, int level, int precision, const void *x, voi
   else
     return ptmf_set_error(PTMF_ERR_UNSUPPORTED, "mg_level_apply: precision %d", precision);
   PTMF_HIP(hipStreamSynchronize(mg->stream));
+  return PTMF_OK;
+}
+
+// Level operator as a CSR matrix on the host (MatAssemble_StokesA_AUU,
+// stokes_assembly.c:115-297, with BCListInsertScaling, dmda_bcs.c:1035-1060):
+// rows and columns in the node-interlaced dof order 3n + c; a Dirichlet row
+// holds its unit diagonal only, a free row the couplings to the free dofs of
+// every node that shares an element with its node, columns ascending.  The
+// values come from the level's 125-point stencil (its Galerkin stencil, or the
+// rediscretised operator probed through its own MatMult with the 375
+// colourings).  With colidx == vals == NULL only *nnz (and rowptr, when not
+// NULL) is computed.
+int ptmf_mg_level_assemble(ptmf_mg mg, int level, long *nnz_out, long *rowptr, long *colidx, double *vals) {
+  PTMF_CHECK(mg && nnz_out && level >= 0 && level < mg->L, PTMF_ERR_ARG, "mg_level_assemble: bad arguments");
+  PTMF_CHECK((colidx == nullptr) == (vals == nullptr), PTMF_ERR_ARG,
+             "mg_level_assemble: colidx and vals are given together");
+  PTMF_CHECK(!colidx || rowptr, PTMF_ERR_ARG, "mg_level_assemble: the fill call needs rowptr");
+  PTMF_CHECK(mg->is_setup, PTMF_ERR_NOT_SETUP, "mg_level_assemble: not set up");
+  ptmf::MGLevel &L = mg->lev[level];
+  const ptmf::Box &b = L.op->box;
+  const long X = b.X, Y = b.Y, Z = b.Z, N = b.nnodes, XY = X * Y;
+  std::vector<uint8_t> bc(N);
+  PTMF_HIP(hipMemcpy(bc.data(), L.op->bcnode, N, hipMemcpyDeviceToHost));
+  // nodes a and a + o of one lattice direction (extent E) share an element
+  // when |o| <= 1, or |o| == 2 with both indices even (element boundaries)
+  auto couples = [](long a, int o, long E) {
+    const long c = a + o;
+    if (c < 0 || c >= E) return false;
+    return o >= -1 && o <= 1 ? true : (a % 2) == 0;
+  };
+  // pass 1: row lengths
+  long nnz = 0;
+  if (rowptr) rowptr[0] = 0;
+  for (long m = 0; m < N; m++) {
+    const long i = m % X, j = (m / X) % Y, k = m / XY;
+    long nfree = 0;  // free dofs of the nodes coupled to m
+    for (int ok = -2; ok <= 2; ok++) {
+      if (!couples(k, ok, Z)) continue;
+      for (int oj = -2; oj <= 2; oj++) {
+        if (!couples(j, oj, Y)) continue;
+        for (int oi = -2; oi <= 2; oi++) {
+          if (!couples(i, oi, X)) continue;
+          const uint8_t bn = bc[m + oi + X * oj + XY * ok];
+          nfree += 3 - ((bn & 1) + ((bn >> 1) & 1) + ((bn >> 2) & 1));
+        }
+      }
+    }
+    for (int r = 0; r < 3; r++) {
+      nnz += ((bc[m] >> r) & 1) ? 1 : nfree;
+      if (rowptr) rowptr[3 * m + r + 1] = nnz;
+    }
+  }
+  *nnz_out = nnz;
+  if (!colidx) return PTMF_OK;
+  // pass 2: the stencil, one offset (9 N values) at a time; offsets in
+  // ascending order give ascending neighbour nodes, so every row comes out
+  // sorted
+  double *S = L.galerkin ? L.S64 : nullptr;
+  if (!S) {
+    PTMF_TRY(ptmf::dalloc((void **)&S, sizeof(double) * 1125 * N));
+    const int st = ptmf::probe_stencil(mg, level, S);
+    if (st != PTMF_OK) {
+      (void)hipFree(S);
+      return st;
+    }
+  }
+  std::vector<long> cur(rowptr, rowptr + 3 * N);
+  std::vector<double> slab(9 * N);
+  int st = PTMF_OK;
+  for (int o = 0; o < 125 && st == PTMF_OK; o++) {
+    const int oi = o % 5 - 2, oj = (o / 5) % 5 - 2, ok = o / 25 - 2;
+    const hipError_t e = hipMemcpy(slab.data(), S + (long)o * 9 * N, sizeof(double) * 9 * N, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      st = ptmf_set_error(PTMF_ERR_DEVICE, "mg_level_assemble: %s", hipGetErrorString(e));
+      break;
+    }
+    for (long m = 0; m < N; m++) {
+      const long i = m % X, j = (m / X) % Y, k = m / XY;
+      if (!couples(i, oi, X) || !couples(j, oj, Y) || !couples(k, ok, Z)) continue;
+      const long n = m + oi + X * oj + XY * ok;
+      for (int r = 0; r < 3; r++) {
+        const long row = 3 * m + r;
+        if ((bc[m] >> r) & 1) {
+          if (o == 62) {  // offset (0, 0, 0): the unit diagonal
+            colidx[cur[row]] = row;
+            vals[cur[row]++] = slab[(3 * r + r) * N + m];
+          }
+          continue;
+        }
+        for (int c = 0; c < 3; c++) {
+          if ((bc[n] >> c) & 1) continue;
+          colidx[cur[row]] = 3 * n + c;
+          vals[cur[row]++] = slab[(3 * r + c) * N + m];
+        }
+      }
+    }
+  }
+  if (!L.galerkin) (void)hipFree(S);
+  PTMF_TRY(st);
+  for (long row = 0; row < 3 * N; row++)
+    PTMF_CHECK(cur[row] == rowptr[row + 1], PTMF_ERR_ARG,
+               "mg_level_assemble: rowptr does not match the level's structure (row %ld)", row);
+  return PTMF_OK;
+}
+
+// The six rigid-body modes of a level from its node coordinates (the near
+// null space the reference attaches to its assembled levels,
+// ptatin_driver_linear_ts.c:663-666): modes[k * 3N + 3n + c], k = 0, 1, 2
+// translations along x, y, z; k = 3 rotation about z (-y, x, 0), k = 4 about
+// x (0, -z, y), k = 5 about y (z, 0, -x).  Host array of 6 * 3N doubles, not
+// normalised.
+int ptmf_mg_level_rigid_body_modes(ptmf_mg mg, int level, double *modes) {
+  PTMF_CHECK(mg && modes && level >= 0 && level < mg->L, PTMF_ERR_ARG, "mg_level_rigid_body_modes: bad arguments");
+  ptmf::MGLevel &L = mg->lev[level];
+  PTMF_CHECK(L.op->has_coords, PTMF_ERR_NOT_SETUP, "mg_level_rigid_body_modes: no coordinates");
+  const long N = L.op->box.nnodes, n3 = 3 * N;
+  std::vector<double> x(n3);
+  PTMF_HIP(hipMemcpy(x.data(), L.op->d64.coords, sizeof(double) * n3, hipMemcpyDeviceToHost));
+  std::fill(modes, modes + 6 * n3, 0.0);
+  for (long n = 0; n < N; n++) {
+    const double px = x[3 * n], py = x[3 * n + 1], pz = x[3 * n + 2];
+    for (int c = 0; c < 3; c++) modes[c * n3 + 3 * n + c] = 1.0;
+    modes[3 * n3 + 3 * n] = -py;
+    modes[3 * n3 + 3 * n + 1] = px;
+    modes[4 * n3 + 3 * n + 1] = -pz;
+    modes[4 * n3 + 3 * n + 2] = py;
+    modes[5 * n3 + 3 * n] = pz;
+    modes[5 * n3 + 3 * n + 2] = -px;
+  }
   return PTMF_OK;
 }
 

@@ -112,6 +112,29 @@ class GMG:
         require("GMG.level_apply", self.level_n(level), x.dtype, x, y)
         call("ptmf_mg_level_apply", self.handle, int(level), prec, x.ptr, y.ptr)
 
+    def level_assemble(self, level: int):
+        """The level's operator as host CSR arrays (rowptr, colidx, vals) in
+        the node-interlaced dof order, the reference's assembled operator
+        (MatAssemble_StokesA_AUU, stokes_assembly.c:115-297, Dirichlet rows
+        and columns replaced by a unit diagonal, dmda_bcs.c:1035-1060)."""
+        n = self.level_n(level)
+        nnz = C.c_long()
+        rowptr = np.zeros(n + 1, dtype=np.int64)
+        call("ptmf_mg_level_assemble", self.handle, int(level), C.byref(nnz), _lib._hp(rowptr), None, None)
+        colidx = np.zeros(nnz.value, dtype=np.int64)
+        vals = np.zeros(nnz.value, dtype=np.float64)
+        call("ptmf_mg_level_assemble", self.handle, int(level), C.byref(nnz), _lib._hp(rowptr), _lib._hp(colidx), _lib._hp(vals))
+        return rowptr, colidx, vals
+
+    def level_rigid_body_modes(self, level: int) -> np.ndarray:
+        """The six rigid-body modes of the level from its coordinates, shape
+        (6, 3N): translations x, y, z, rotations about z, x, y (the near null
+        space of ptatin_driver_linear_ts.c:663-666)."""
+        n = self.level_n(level)
+        modes = np.zeros((6, n), dtype=np.float64)
+        call("ptmf_mg_level_rigid_body_modes", self.handle, int(level), _lib._hp(modes))
+        return modes
+
     def vcycle(self, b: DeviceArray, x: DeviceArray) -> None:
         require("GMG.vcycle", self.n, np.float64, b, x)
         call("ptmf_mg_vcycle", self.handle, b.ptr, x.ptr)
