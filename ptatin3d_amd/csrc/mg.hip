@@ -375,66 +375,67 @@ __global__ void k_lsgs_update(long n, int k, double *const *__restrict__ V, cons
 // ---------------------------------------------------------------------------
 // yf += P xc: fine node I takes coarse node I/2 (I even) or the mean of
 // coarse nodes (I-1)/2 and (I+1)/2 (I odd), tensor product, 3 components.
+// Launch: blockIdx.y = fine plane K, threads over the plane's Xf * Yf nodes
+// (one 32-bit division per node; 32-bit indices, the host checks 3 N fits).
 template <typename T>
-__global__ void k_prolong_add(long Xc, long Yc, long Xf, long Yf, long Zf, const T *__restrict__ xc,
-                              T *__restrict__ yf) {
-  const long nf = Xf * Yf * Zf;
-  for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < nf; f += (long)gridDim.x * blockDim.x) {
-    const long I = f % Xf, J = (f / Xf) % Yf, K = f / (Xf * Yf);
-    long ci[2], cj[2], ck[2];
-    T wi[2], wj[2], wk[2];
-    int ni, nj, nk;
-    if (I & 1) { ci[0] = I >> 1; ci[1] = ci[0] + 1; wi[0] = wi[1] = T(0.5); ni = 2; } else { ci[0] = I >> 1; wi[0] = T(1); ni = 1; }
-    if (J & 1) { cj[0] = J >> 1; cj[1] = cj[0] + 1; wj[0] = wj[1] = T(0.5); nj = 2; } else { cj[0] = J >> 1; wj[0] = T(1); nj = 1; }
-    if (K & 1) { ck[0] = K >> 1; ck[1] = ck[0] + 1; wk[0] = wk[1] = T(0.5); nk = 2; } else { ck[0] = K >> 1; wk[0] = T(1); nk = 1; }
-    T s0 = 0, s1 = 0, s2 = 0;
-    for (int c = 0; c < nk; c++)
-      for (int b = 0; b < nj; b++)
-        for (int a = 0; a < ni; a++) {
-          const long g = ci[a] + Xc * (cj[b] + Yc * ck[c]);
-          const T w = wi[a] * wj[b] * wk[c];
-          s0 += w * xc[3 * g];
-          s1 += w * xc[3 * g + 1];
-          s2 += w * xc[3 * g + 2];
-        }
-    yf[3 * f] += s0;
-    yf[3 * f + 1] += s1;
-    yf[3 * f + 2] += s2;
-  }
+__global__ void k_prolong_add(int Xc, int Yc, int Xf, int Yf, const T *__restrict__ xc, T *__restrict__ yf) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Xf * Yf) return;
+  const int J = t / Xf, I = t - J * Xf, K = blockIdx.y;
+  int ci[2], cj[2], ck[2];
+  T wi[2], wj[2], wk[2];
+  int ni, nj, nk;
+  if (I & 1) { ci[0] = I >> 1; ci[1] = ci[0] + 1; wi[0] = wi[1] = T(0.5); ni = 2; } else { ci[0] = I >> 1; wi[0] = T(1); ni = 1; }
+  if (J & 1) { cj[0] = J >> 1; cj[1] = cj[0] + 1; wj[0] = wj[1] = T(0.5); nj = 2; } else { cj[0] = J >> 1; wj[0] = T(1); nj = 1; }
+  if (K & 1) { ck[0] = K >> 1; ck[1] = ck[0] + 1; wk[0] = wk[1] = T(0.5); nk = 2; } else { ck[0] = K >> 1; wk[0] = T(1); nk = 1; }
+  T s0 = 0, s1 = 0, s2 = 0;
+  for (int c = 0; c < nk; c++)
+    for (int b = 0; b < nj; b++)
+      for (int a = 0; a < ni; a++) {
+        const int g = 3 * (ci[a] + Xc * (cj[b] + Yc * ck[c]));
+        const T w = wi[a] * wj[b] * wk[c];
+        s0 += w * xc[g];
+        s1 += w * xc[g + 1];
+        s2 += w * xc[g + 2];
+      }
+  const int f = 3 * (t + Xf * Yf * K);
+  yf[f] += s0;
+  yf[f + 1] += s1;
+  yf[f + 2] += s2;
 }
 
 // yc = P^T xf, gathered per coarse node from fine nodes 2i-1, 2i, 2i+1
-// (weights 1/2, 1, 1/2 per direction).
+// (weights 1/2, 1, 1/2 per direction).  Launch as k_prolong_add, over the
+// coarse planes.
 template <typename T>
-__global__ void k_restrict(long Xc, long Yc, long Zc, long Xf, long Yf, long Zf, const T *__restrict__ xf,
-                           T *__restrict__ yc) {
-  const long ncn = Xc * Yc * Zc;
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < ncn; g += (long)gridDim.x * blockDim.x) {
-    const long i = g % Xc, j = (g / Xc) % Yc, k = g / (Xc * Yc);
-    T s0 = 0, s1 = 0, s2 = 0;
-    for (int dk = -1; dk <= 1; dk++) {
-      const long K = 2 * k + dk;
-      if (K < 0 || K >= Zf) continue;
-      const T wk = dk == 0 ? T(1) : T(0.5);
-      for (int dj = -1; dj <= 1; dj++) {
-        const long J = 2 * j + dj;
-        if (J < 0 || J >= Yf) continue;
-        const T wj = dj == 0 ? T(1) : T(0.5);
-        for (int di = -1; di <= 1; di++) {
-          const long I = 2 * i + di;
-          if (I < 0 || I >= Xf) continue;
-          const T w = (di == 0 ? T(1) : T(0.5)) * wj * wk;
-          const long f = I + Xf * (J + Yf * K);
-          s0 += w * xf[3 * f];
-          s1 += w * xf[3 * f + 1];
-          s2 += w * xf[3 * f + 2];
-        }
+__global__ void k_restrict(int Xc, int Yc, int Xf, int Yf, int Zf, const T *__restrict__ xf, T *__restrict__ yc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Xc * Yc) return;
+  const int j = t / Xc, i = t - j * Xc, k = blockIdx.y;
+  T s0 = 0, s1 = 0, s2 = 0;
+  for (int dk = -1; dk <= 1; dk++) {
+    const int K = 2 * k + dk;
+    if (K < 0 || K >= Zf) continue;
+    const T wk = dk == 0 ? T(1) : T(0.5);
+    for (int dj = -1; dj <= 1; dj++) {
+      const int J = 2 * j + dj;
+      if (J < 0 || J >= Yf) continue;
+      const T wj = dj == 0 ? T(1) : T(0.5);
+      for (int di = -1; di <= 1; di++) {
+        const int I = 2 * i + di;
+        if (I < 0 || I >= Xf) continue;
+        const T w = (di == 0 ? T(1) : T(0.5)) * wj * wk;
+        const int f = 3 * (I + Xf * (J + Yf * K));
+        s0 += w * xf[f];
+        s1 += w * xf[f + 1];
+        s2 += w * xf[f + 2];
       }
     }
-    yc[3 * g] = s0;
-    yc[3 * g + 1] = s1;
-    yc[3 * g + 2] = s2;
   }
+  const int g = 3 * (t + Xc * Yc * k);
+  yc[g] = s0;
+  yc[g + 1] = s1;
+  yc[g + 2] = s2;
 }
 
 // Q2 embedding (transfer option 1): the fine nodal values of the coarse Q2
@@ -1065,10 +1066,13 @@ int matmult(const MGLevel &L, const T *x, T *y) {
 template <typename T>
 int prolong_add(ptmf_mg_s *mg, int l, const T *xc, T *yf) {
   const Box &bf = mg->lev[l].op->box, &bc = mg->lev[l - 1].op->box;
-  if (mg->opt.transfer == 1)
+  if (mg->opt.transfer == 1) {
     k_prolong_add_q2<T><<<nblocks(bf.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bf.X, bf.Y, bf.Z, xc, yf);
-  else
-    k_prolong_add<T><<<nblocks(bf.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bf.X, bf.Y, bf.Z, xc, yf);
+  } else {
+    PTMF_CHECK(3 * bf.nnodes < (1L << 31), PTMF_ERR_UNSUPPORTED, "prolong: level too large for 32-bit indices");
+    const dim3 grid((unsigned)((bf.X * bf.Y + kBS - 1) / kBS), (unsigned)bf.Z);
+    k_prolong_add<T><<<grid, kBS, 0, mg->stream>>>((int)bc.X, (int)bc.Y, (int)bf.X, (int)bf.Y, xc, yf);
+  }
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }
@@ -1076,10 +1080,13 @@ int prolong_add(ptmf_mg_s *mg, int l, const T *xc, T *yf) {
 template <typename T>
 int restrict_(ptmf_mg_s *mg, int l, const T *xf, T *yc) {
   const Box &bf = mg->lev[l].op->box, &bc = mg->lev[l - 1].op->box;
-  if (mg->opt.transfer == 1)
+  if (mg->opt.transfer == 1) {
     k_restrict_q2<T><<<nblocks(bc.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z, xf, yc);
-  else
-    k_restrict<T><<<nblocks(bc.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z, xf, yc);
+  } else {
+    PTMF_CHECK(3 * bf.nnodes < (1L << 31), PTMF_ERR_UNSUPPORTED, "restrict: level too large for 32-bit indices");
+    const dim3 grid((unsigned)((bc.X * bc.Y + kBS - 1) / kBS), (unsigned)bc.Z);
+    k_restrict<T><<<grid, kBS, 0, mg->stream>>>((int)bc.X, (int)bc.Y, (int)bf.X, (int)bf.Y, (int)bf.Z, xf, yc);
+  }
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
 }
