@@ -373,15 +373,28 @@ __global__ void k_lsgs_update(long n, int k, double *const *__restrict__ V, cons
 // ---------------------------------------------------------------------------
 // transfers between level l-1 (coarse, mxc x myc x mzc) and l (fine, doubled)
 // ---------------------------------------------------------------------------
+// Work-groups of a 1-D launch go round-robin to the 8 XCDs (each with its
+// own L2).  Work-group b of `total` (b-th in dispatch order) takes item
+// xcd_item(b): every XCD gets one contiguous run of items, so lattice planes
+// that neighbouring items share are fetched into one L2, not into several.
+__device__ __forceinline__ int xcd_item(int b, int total) {
+  const int per = total / 8, rem = total % 8, x = b % 8, slot = b / 8;
+  return x * per + (x < rem ? x : rem) + slot;
+}
+
 // yf += P xc: fine node I takes coarse node I/2 (I even) or the mean of
 // coarse nodes (I-1)/2 and (I+1)/2 (I odd), tensor product, 3 components.
-// Launch: blockIdx.y = fine plane K, threads over the plane's Xf * Yf nodes
-// (one 32-bit division per node; 32-bit indices, the host checks 3 N fits).
+// Launch: bpp work-groups per fine plane, planes in z order, items mapped to
+// XCDs by xcd_item (one 32-bit division per node; 32-bit indices, the host
+// checks 3 N fits).
 template <typename T>
-__global__ void k_prolong_add(int Xc, int Yc, int Xf, int Yf, const T *__restrict__ xc, T *__restrict__ yf) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_prolong_add(int Xc, int Yc, int Xf, int Yf, int bpp, const T *__restrict__ xc,
+                              T *__restrict__ yf) {
+  const int q = xcd_item(blockIdx.x, gridDim.x);
+  const int K = q / bpp;
+  const int t = (q - K * bpp) * blockDim.x + threadIdx.x;
   if (t >= Xf * Yf) return;
-  const int J = t / Xf, I = t - J * Xf, K = blockIdx.y;
+  const int J = t / Xf, I = t - J * Xf;
   int ci[2], cj[2], ck[2];
   T wi[2], wj[2], wk[2];
   int ni, nj, nk;
@@ -408,10 +421,13 @@ __global__ void k_prolong_add(int Xc, int Yc, int Xf, int Yf, const T *__restric
 // (weights 1/2, 1, 1/2 per direction).  Launch as k_prolong_add, over the
 // coarse planes.
 template <typename T>
-__global__ void k_restrict(int Xc, int Yc, int Xf, int Yf, int Zf, const T *__restrict__ xf, T *__restrict__ yc) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_restrict(int Xc, int Yc, int Xf, int Yf, int Zf, int bpp, const T *__restrict__ xf,
+                           T *__restrict__ yc) {
+  const int q = xcd_item(blockIdx.x, gridDim.x);
+  const int k = q / bpp;
+  const int t = (q - k * bpp) * blockDim.x + threadIdx.x;
   if (t >= Xc * Yc) return;
-  const int j = t / Xc, i = t - j * Xc, k = blockIdx.y;
+  const int j = t / Xc, i = t - j * Xc;
   T s0 = 0, s1 = 0, s2 = 0;
   for (int dk = -1; dk <= 1; dk++) {
     const int K = 2 * k + dk;
@@ -1070,8 +1086,9 @@ int prolong_add(ptmf_mg_s *mg, int l, const T *xc, T *yf) {
     k_prolong_add_q2<T><<<nblocks(bf.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bf.X, bf.Y, bf.Z, xc, yf);
   } else {
     PTMF_CHECK(3 * bf.nnodes < (1L << 31), PTMF_ERR_UNSUPPORTED, "prolong: level too large for 32-bit indices");
-    const dim3 grid((unsigned)((bf.X * bf.Y + kBS - 1) / kBS), (unsigned)bf.Z);
-    k_prolong_add<T><<<grid, kBS, 0, mg->stream>>>((int)bc.X, (int)bc.Y, (int)bf.X, (int)bf.Y, xc, yf);
+    const int bpp = (int)((bf.X * bf.Y + kBS - 1) / kBS);
+    k_prolong_add<T><<<dim3((unsigned)(bpp * bf.Z)), kBS, 0, mg->stream>>>((int)bc.X, (int)bc.Y, (int)bf.X,
+                                                                          (int)bf.Y, bpp, xc, yf);
   }
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
@@ -1084,8 +1101,9 @@ int restrict_(ptmf_mg_s *mg, int l, const T *xf, T *yc) {
     k_restrict_q2<T><<<nblocks(bc.nnodes), kBS, 0, mg->stream>>>(bc.X, bc.Y, bc.Z, bf.X, bf.Y, bf.Z, xf, yc);
   } else {
     PTMF_CHECK(3 * bf.nnodes < (1L << 31), PTMF_ERR_UNSUPPORTED, "restrict: level too large for 32-bit indices");
-    const dim3 grid((unsigned)((bc.X * bc.Y + kBS - 1) / kBS), (unsigned)bc.Z);
-    k_restrict<T><<<grid, kBS, 0, mg->stream>>>((int)bc.X, (int)bc.Y, (int)bf.X, (int)bf.Y, (int)bf.Z, xf, yc);
+    const int bpp = (int)((bc.X * bc.Y + kBS - 1) / kBS);
+    k_restrict<T><<<dim3((unsigned)(bpp * bc.Z)), kBS, 0, mg->stream>>>((int)bc.X, (int)bc.Y, (int)bf.X, (int)bf.Y,
+                                                                       (int)bf.Z, bpp, xf, yc);
   }
   PTMF_HIP(hipGetLastError());
   return PTMF_OK;
