@@ -14,8 +14,9 @@ timed region.  Further objects of the line:
           contrast-1e6 workload at --solve-m elements per direction (default
           128): flexible CG in FP64 with a 4-level V-cycle (Chebyshev(3),
           exact FP64 coarse factor), FP32 V-cycle against FP64 V-cycle, both
-          to a 1e-8 FP64 residual and compared with a 1e-11 FP64 solve
-          (--no-solve skips it);
+          to a 1e-8 FP64 residual and compared with a 1e-11 FP64 solve, with
+          the heuristic's tilings (--solve-tune: every level's tiling from the
+          autotuner in its V-cycle's precision; --no-solve skips the leg);
   stokes  the rift saddle point (--stokes MX MY MZ, default 192 192 96;
           --no-stokes skips it): fused MatMult rate, then FP32- and
           FP64-V-cycle FGMRES solves to 1e-8 and their error against a 1e-11
@@ -27,7 +28,7 @@ multigrid solve of --batch-solve boxes (default 256; 0 skips it) against
 each box solved on its own.
 
   python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--m 64] [--no-cpu]
-                  [--solve-m 128 | --no-solve] [--copy-probe] [--batch-solve 256 | --batch-solve 0]
+                  [--solve-m 128 | --no-solve] [--solve-tune] [--copy-probe] [--batch-solve 256 | --batch-solve 0]
                   [--stokes 192 192 96 | --no-stokes] [--stokes-schur bdb]
 """
 from __future__ import annotations
@@ -91,6 +92,8 @@ def main() -> int:
                     help="outer Krylov method of the solve leg (FGMRES(30) stagnates near 0.9 on this workload)")
     ap.add_argument("--solve-restart", type=int, default=30)
     ap.add_argument("--solve-galerkin", type=int, default=0, help="Galerkin coarse levels of the solve leg")
+    ap.add_argument("--solve-tune", action="store_true",
+                    help="solve leg with every level's tiling from the autotuner (default: the heuristic's)")
     ap.add_argument("--solve-maxit", type=int, default=4000,
                     help="iteration cap of each solve of the solve leg (a few for a profiled run)")
     ap.add_argument("--copy-probe", action="store_true")
@@ -302,7 +305,8 @@ def main() -> int:
         out["batch"] = batch_leg(args.batch)
     if not args.no_solve:
         out["solve"] = solve_leg(args.solve_m, args.solve_levels, args.solve_cheb, args.solve_krylov,
-                                 args.solve_restart, args.solve_galerkin, args.solve_maxit)
+                                 args.solve_restart, args.solve_galerkin, args.solve_maxit,
+                                 tune=args.solve_tune)
     if args.batch_solve > 0:
         out["batch_solve"] = batch_solve_leg(args.batch_solve)
     if args.stokes and not args.no_stokes:
@@ -510,7 +514,7 @@ def batch_solve_leg(nbox: int) -> dict:
 
 
 def solve_leg(m: int, levels: int = 4, cheb: int = 3, krylov: str = "fgmres", restart: int = 30,
-              galerkin: int = 0, maxit: int = 4000) -> dict:
+              galerkin: int = 0, maxit: int = 4000, tune: bool = False) -> dict:
     """Layered viscosity 1e3/1e1/1e-3 with 64 weak ellipsoids (contrast 1e6),
     smooth body force, all-face Dirichlet, on m^3 Q2 elements.  The A11 solve
     of the configured workload: FGMRES(restart) in FP64 (or flexible CG)
@@ -522,8 +526,11 @@ def solve_leg(m: int, levels: int = 4, cheb: int = 3, krylov: str = "fgmres", re
     The V-cycle (operators, smoother, transfers) runs in FP32 or in FP64;
     both solves go to ||b - A x|| <= 1e-8 ||b|| (FP64 residual of the FP64
     operator); each solution is compared with a 1e-11 FP64 solve (the FP64
-    run continued from its own solution)."""
-    from ptatin3d_amd import fields as F
+    run continued from its own solution).  With `tune`, every level's tile
+    shape, kernel variant and tiles per work-group come from the autotuner,
+    timed in the precision the V-cycle applies that level in (set-up work,
+    outside the timed solve)."""
+    from ptatin3d_amd import autotune, fields as F
     from ptatin3d_amd._lib import DeviceArray, call
     from ptatin3d_amd.multigrid import GMG, default_options, solve_timed
 
@@ -538,7 +545,9 @@ def solve_leg(m: int, levels: int = 4, cheb: int = 3, krylov: str = "fgmres", re
             "after 4000 iterations at 128^3; 7e-3 after 1500 at 64^3) and FGMRES(100) reaches 1e-4 after 1500 at 64^3"),
            "smoother": f"Chebyshev({cheb})-Jacobi on [0.2, 1.1] lambda_max",
            "transfers": "DMDA Q1 lattice interpolation, R = P^T",
-           "coarse_solve": "FP64 multifrontal Cholesky of the assembled coarsest level (exact)"}
+           "coarse_solve": "FP64 multifrontal Cholesky of the assembled coarsest level (exact)",
+           "level_tiling": ("every level autotuned in its V-cycle's precision; per run, level: [TX, TY, ZC, "
+                            "kernel variant, tiles per work-group]" if tune else "the heuristic's")}
     runs = {}
     bnorm = float(np.linalg.norm(c["b"]))
     for tag, prec in (("fp64", 64), ("fp32", 32)):
@@ -552,6 +561,11 @@ def solve_leg(m: int, levels: int = 4, cheb: int = 3, krylov: str = "fgmres", re
                                  coarse_factor=2))
         ts = time.perf_counter() - t0
         info = mg.coarse_info()
+        tuned, tt = None, 0.0
+        if tune:
+            t0 = time.perf_counter()
+            tuned = autotune.tune_hierarchy(mg, precision=prec)
+            tt = time.perf_counter() - t0
         # the V-cycle alone (the part the smoother precision changes)
         db, dv = DeviceArray.from_host(c["b"]), DeviceArray(c["b"].size)
         mg.vcycle(db, dv)
@@ -569,7 +583,10 @@ def solve_leg(m: int, levels: int = 4, cheb: int = 3, krylov: str = "fgmres", re
                     "rel_residual": float(rn / bnorm), "vcycle_ms": round(vms, 3),
                     "outer_ms_per_it": round(1e3 * secs / max(its, 1) - vms, 3),
                     "coarse_factor_MB": round(info["bytes"] / 1e6, 1),
-                    "coarse_factor_ms": round(info["setup_ms"], 1)}
+                    "coarse_factor_ms": round(info["setup_ms"], 1),
+                    "tiling": ({f"L{d['level']}": d["tiling"] + [d["variant"], d["tiles_per_group"]] for d in tuned}
+                               if tuned else "heuristic"),
+                    "tuning_seconds": round(tt, 2)}
         print(f"# solve leg {tag}: {its} its, {secs:.2f} s, rel residual {rn / bnorm:.2e}",
               file=sys.stderr, flush=True)
         if tag == "fp64":
